@@ -1,0 +1,314 @@
+"""Python mirror of the reference's ADSS / Setup / Interfaces module API.
+
+The reference (macwozni/ADS_MPI) drives a problem as
+
+    call initialize(n, p, ads, ads_data, mierr)          ! src/ADS.F90:11
+    do iter = 0, steps
+       call Step(iter, ComputePointForRHS, ads, ads_data, l2norm, mierr)   ! :437
+       t = t + Dt
+    enddo
+    call Cleanup(ads, ads_data, mierr)                   ! :802
+
+(problems/heat/main.F90:46-62).  The same sequence works here:
+
+    ads, ads_data = initialize(n, p)
+    rhs = HeatRHS(Dt=1e-5, f0=ADS_F0_SINPROD)
+    for it in range(steps + 1):
+        l2norm = Step(it, rhs, ads, ads_data)
+        rhs.t += rhs.Dt
+    Cleanup(ads, ads_data)
+
+The per-point Fortran callback RHS_fun (Interfaces.F90:52-81) cannot run on
+the device, so integrands are *registered forms*: HeatRHS restates
+problems/heat/RHS_eq.F90:35-88 (t == 0 projection of f0, t > 0 explicit heat
+step) and L2ProjectionRHS restates problems/L2Projection/RHS_eq.F90:34-71.
+Anything else raises, as the Fortran drop-in returns mierr != 0.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ADS_F0_BUMP3D, ADS_F0_CONST, ADS_F0_SINPROD, ADS_PREC_FP32_IR, ADS_PREC_FP64, AdsError,
+                   check)
+
+__all__ = [
+    "ADS_setup", "ADS_compute_data", "initialize", "Step", "MultiStep", "Cleanup", "Form3DRHS", "FormUn",
+    "ComputeMatrix", "HeatRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
+    "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+@dataclass
+class ADS_setup:
+    """Host mirror of type(ADS_setup) (src/Setup.F90:29-112) on one rank."""
+
+    n: np.ndarray
+    p: np.ndarray
+    nelem: np.ndarray
+    m: np.ndarray
+    ng: np.ndarray
+    Ux: np.ndarray
+    Uy: np.ndarray
+    Uz: np.ndarray
+    Ox: np.ndarray
+    Oy: np.ndarray
+    Oz: np.ndarray
+    Jx: np.ndarray
+    Jy: np.ndarray
+    Jz: np.ndarray
+    Wx: np.ndarray
+    Wy: np.ndarray
+    Wz: np.ndarray
+    Xx: np.ndarray  # (nelem, q): Fortran X(q, nelem)
+    Xy: np.ndarray
+    Xz: np.ndarray
+    NNx: np.ndarray  # (nelem, q, p+1, 2): Fortran NN(0:1, 0:p, q, nelem)
+    NNy: np.ndarray
+    NNz: np.ndarray
+    # one-rank decomposition values (parallelism.F90:146-171 with rank 0 of 1)
+    s: np.ndarray = field(default=None)
+    ibeg: np.ndarray = field(default=None)
+    iend: np.ndarray = field(default=None)
+    mine: np.ndarray = field(default=None)
+    maxe: np.ndarray = field(default=None)
+    lnelem: np.ndarray = field(default=None)
+    nrcpp: np.ndarray = field(default=None)
+    tau: float = 0.0
+
+    @property
+    def dofs(self):
+        return tuple(int(v) + 1 for v in self.n)
+
+
+class ADS_compute_data:
+    """type(ADS_compute_data) (src/Setup.F90:120-140): the solver state.
+
+    The coefficients live on the GPU; ``F`` copies them to the host when read
+    (the reference keeps them in ads_data%F, x fastest)."""
+
+    def __init__(self, handle, D, dofs):
+        self.handle = handle
+        self.D = D
+        self.dofs = dofs
+        self.t = 0.0
+
+    @property
+    def F(self):
+        out = np.empty(self.D)
+        check(_lib.load().ads_get_coeffs(self.handle, _d(out)), self.handle)
+        return out
+
+    @F.setter
+    def F(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        if v.size != self.D:
+            raise ValueError("expected %d coefficients, got %d" % (self.D, v.size))
+        check(_lib.load().ads_set_coeffs(self.handle, _d(v)), self.handle)
+
+    def coeffs3d(self):
+        """Coefficients as an array [z][y][x] (x fastest in memory)."""
+        nx, ny, nz = self.dofs
+        return self.F.reshape(nz, ny, nx)
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        check(_lib.load().ads_coeffs_device(self.handle, ctypes.byref(p)), self.handle)
+        return p.value
+
+
+class HeatRHS:
+    """problems/heat/RHS_eq.F90:35-88 as a device form.
+
+    t == 0: ret = J W v f0(X), l2norm = J W v f0^2 (projection of the initial
+    state); t > 0: ret = J W (v u + Dt(-kappa grad u . grad v)), l2norm =
+    J W v u^2.  ``t`` is advanced by the driver, as main.F90:57 does."""
+
+    def __init__(self, Dt, f0=ADS_F0_BUMP3D, kappa=None, prec=ADS_PREC_FP64, refine=2):
+        self.Dt = float(Dt)
+        self.f0 = int(f0)
+        self.kappa = kappa
+        self.prec = int(prec)
+        self.refine = int(refine)
+        self.t = 0.0
+
+
+class L2ProjectionRHS:
+    """problems/L2Projection/RHS_eq.F90:34-71: f = 1."""
+
+    t = 0.0
+    f0 = ADS_F0_CONST
+
+
+def _direction(L, h, d, n, p):
+    q = p + 1
+    ne = n - p + 1
+    U = np.zeros(n + p + 2)
+    O = np.zeros(ne, dtype=np.int32)
+    J = np.zeros(ne)
+    W = np.zeros(q)
+    X = np.zeros((ne, q))
+    NN = np.zeros((ne, q, p + 1, 2))
+    check(L.ads_get_knots(h, d, _d(U)), h)
+    check(L.ads_get_basis(h, d, _i(O), _d(J), _d(W), _d(X), _d(NN)), h)
+    return U, O, J, W, X, NN
+
+
+def initialize(n, p):
+    """ADSS::initialize (src/ADS.F90:11-107).  Returns (ads, ads_data)."""
+    L = _lib.load()
+    n3 = (ctypes.c_int * 3)(*([int(n)] * 3 if np.isscalar(n) else [int(v) for v in n]))
+    p3 = (ctypes.c_int * 3)(*([int(p)] * 3 if np.isscalar(p) else [int(v) for v in p]))
+    h = ctypes.c_void_p()
+    rc = L.ads_init(n3, p3, ctypes.byref(h))
+    if rc != 0:
+        raise AdsError(rc, L.ads_last_error(None).decode())
+    nn = np.array(list(n3), dtype=np.int32)
+    pp = np.array(list(p3), dtype=np.int32)
+    tabs = [_direction(L, h, d, int(nn[d]), int(pp[d])) for d in range(3)]
+    nelem = nn - pp + 1
+    ads = ADS_setup(
+        n=nn, p=pp, nelem=nelem, m=nn + pp + 1, ng=pp + 1,
+        Ux=tabs[0][0], Uy=tabs[1][0], Uz=tabs[2][0],
+        Ox=tabs[0][1], Oy=tabs[1][1], Oz=tabs[2][1],
+        Jx=tabs[0][2], Jy=tabs[1][2], Jz=tabs[2][2],
+        Wx=tabs[0][3], Wy=tabs[1][3], Wz=tabs[2][3],
+        Xx=tabs[0][4], Xy=tabs[1][4], Xz=tabs[2][4],
+        NNx=tabs[0][5], NNy=tabs[1][5], NNz=tabs[2][5],
+        s=nn + 1, ibeg=np.ones(3, dtype=np.int32), iend=nn + 1, mine=np.ones(3, dtype=np.int32),
+        maxe=nelem, lnelem=nelem, nrcpp=nn + 1,
+    )
+    D = int(np.prod(nn + 1))
+    data = ADS_compute_data(h, D, tuple(int(v) + 1 for v in nn))
+    return ads, data
+
+
+def _set_kappa(ads_data, kappa):
+    L = _lib.load()
+    if kappa is None:
+        check(L.ads_set_kappa(ads_data.handle, None), ads_data.handle)
+    else:
+        k = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
+        check(L.ads_set_kappa(ads_data.handle, _d(k)), ads_data.handle)
+
+
+def Step(iter, RHS_fun, ads, ads_data):
+    """ADSS::Step (src/ADS.F90:437-461) with a registered integrand form.
+
+    Returns l2norm (RHS_eq.F90:81,85).  The step replaces the coefficients in
+    ads_data; ``iter`` is accepted for interface parity (the integrand decides
+    between projection and time step from its own ``t``)."""
+    L = _lib.load()
+    h = ads_data.handle
+    out = ctypes.c_double(0.0)
+    if isinstance(RHS_fun, L2ProjectionRHS):
+        check(L.ads_project(h, ADS_F0_CONST, ctypes.byref(out)), h)
+        return out.value
+    if not isinstance(RHS_fun, HeatRHS):
+        raise AdsError(_lib.ADS_ERR_UNSUPPORTED,
+                       "RHS_fun is not a registered device form (HeatRHS, L2ProjectionRHS)")
+    if RHS_fun.t == 0.0:
+        check(L.ads_project(h, RHS_fun.f0, ctypes.byref(out)), h)
+    else:
+        _set_kappa(ads_data, RHS_fun.kappa)
+        check(L.ads_step(h, RHS_fun.Dt, RHS_fun.prec, RHS_fun.refine, ctypes.byref(out)), h)
+    ads_data.t = RHS_fun.t
+    return out.value
+
+
+def MultiStep(iter, mix, RHS_fun, ads, ads_data):
+    """ADSS::MultiStep (src/ADS.F90:391-429): the alternating-direction
+    substeps with per-direction operators are not part of this build."""
+    raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "MultiStep (mix-based ADS substeps) is not implemented")
+
+
+def Form3DRHS(ads, ads_data, RHS_fun):
+    """projection_engine::Form3DRHS after FormUn (projection_engine.F90:154-284)
+    for the heat integrand at t > 0.  Returns (F, l2norm) on the host without
+    changing the state."""
+    import torch  # device buffers
+
+    L = _lib.load()
+    h = ads_data.handle
+    if not isinstance(RHS_fun, HeatRHS) or RHS_fun.t == 0.0:
+        raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "Form3DRHS: only the t > 0 heat form is exposed")
+    F = torch.empty(ads_data.D, dtype=torch.float64, device="cuda")
+    l2 = torch.empty(1, dtype=torch.float64, device="cuda")
+    kap = None
+    if RHS_fun.kappa is not None:
+        kap = torch.as_tensor(np.ascontiguousarray(RHS_fun.kappa, dtype=np.float64), device="cuda")
+    torch.cuda.synchronize()
+    check(L.ads_form_rhs_d(h, ctypes.c_void_p(ads_data.device_ptr()),
+                           ctypes.c_void_p(kap.data_ptr()) if kap is not None else None,
+                           RHS_fun.Dt, ctypes.c_void_p(F.data_ptr()), ctypes.c_void_p(l2.data_ptr())), h)
+    check(L.ads_synchronize(h), h)
+    return F.cpu().numpy(), float(l2.cpu()[0])
+
+
+def FormUn(subun, ads, ads_data):
+    """projection_engine::FormUn (:306-441) is fused into the assembly kernel:
+    values and gradients at the Gauss points are never stored."""
+    raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "FormUn is fused into Form3DRHS on the GPU")
+
+
+def ComputeMatrix(ads, ads_data, direction, mix):
+    """projection_engine::ComputeMatrix / MKBBT (:531-544, :63-132): the dense
+    1-D matrix mix(1)M + mix(2)K + mix(3)B + mix(4)B^T of one direction."""
+    L = _lib.load()
+    N = int(ads.n[direction]) + 1
+    out = np.zeros((N, N))
+    m = np.asarray(mix, dtype=np.float64)
+    check(L.ads_get_matrix(ads_data.handle, int(direction), _d(m), _d(out)), ads_data.handle)
+    return out
+
+
+def Cleanup(ads, ads_data):
+    """ADSS::Cleanup (src/ADS.F90:802-863)."""
+    if ads_data.handle:
+        _lib.load().ads_destroy(ads_data.handle)
+        ads_data.handle = None
+    return 0
+
+
+def uniform(seed, n, lo, hi):
+    """The library's SplitMix64 uniform generator (ads_fill_uniform)."""
+    out = np.zeros(int(n))
+    check(_lib.load().ads_fill_uniform(int(seed), int(n), float(lo), float(hi), _d(out)))
+    return out
+
+
+def lognormal(seed, n, mu, sigma):
+    out = np.zeros(int(n))
+    check(_lib.load().ads_fill_lognormal(int(seed), int(n), float(mu), float(sigma), _d(out)))
+    return out
+
+
+def host_basis(n, p):
+    """Tables of one direction computed on the host only (ads_host_basis)."""
+    L = _lib.load()
+    q = p + 1
+    ne = n - p + 1
+    if ne < 1 or p < 1:
+        check(L.ads_host_basis(int(n), int(p), None, None, None, None, None, None))
+    U = np.zeros(n + p + 2)
+    O = np.zeros(ne, dtype=np.int32)
+    J = np.zeros(ne)
+    W = np.zeros(q)
+    X = np.zeros((ne, q))
+    NN = np.zeros((ne, q, p + 1, 2))
+    check(L.ads_host_basis(int(n), int(p), _d(U), _i(O), _d(J), _d(W), _d(X), _d(NN)))
+    return U, O, J, W, X, NN

@@ -1,0 +1,516 @@
+// capi.cpp -- the C ABI (include/ads_mi355x.h): context, device state,
+// stream ordering and the step orchestration (Step / Sub_Step of
+// src/ADS.F90:437-794 on one GPU).
+#include "ads_mi355x.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "setup.hpp"
+
+namespace {
+
+thread_local std::string g_init_error;
+
+template <typename T>
+struct DevDir {
+    int* first_dof = nullptr;
+    T* tab = nullptr;
+    T* jac = nullptr;
+    T* wts = nullptr;
+    T* pts = nullptr;
+    T* fwd = nullptr;
+    int* piv = nullptr;
+    T* bwd = nullptr;
+};
+
+}  // namespace
+
+struct ads_ctx {
+    std::string err;
+    int p = 0;
+    ads::Direction dir[3];
+    long D = 0;
+    long E = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+
+    double* c = nullptr;      // state coefficients (x fastest)
+    double* F = nullptr;      // FP64 right-hand side / solve buffer
+    double* kappa = nullptr;  // per-element diffusivity or nullptr (kappa = 1)
+    double* l2_part = nullptr;
+    int l2_cap = 0;
+    double* l2 = nullptr;  // device scalar: last l2norm
+
+    DevDir<double> d64[3];
+    DevDir<float> d32[3];
+    double* mass[3] = {nullptr, nullptr, nullptr};
+    double* stiff[3] = {nullptr, nullptr, nullptr};
+    ads::Tables<double> tb64{};
+    ads::Tables<float> tb32{};
+    ads::SweepTables<double> sw64[3]{};
+    ads::SweepTables<float> sw32[3]{};
+
+    // lazily allocated scratch
+    float* f32a = nullptr;
+    float* f32b = nullptr;
+    double* x64 = nullptr;
+    double* w0 = nullptr;
+    double* w1 = nullptr;
+    double* w2 = nullptr;  // 2*D
+    double* mu = nullptr;
+
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(ads_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    else g_init_error = msg;
+    return code;
+}
+
+int hip_fail(ads_ctx* ctx, hipError_t e, const char* what)
+{
+    return fail(ctx, ADS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define ADS_HIP(ctx, call)                                        \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #call);  \
+    } while (0)
+
+template <typename T>
+int dev_alloc(ads_ctx* ctx, T** out, size_t count)
+{
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, ADS_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
+                                            " bytes failed: " + hipGetErrorString(e));
+    }
+    ctx->allocs.push_back(p);
+    *out = static_cast<T*>(p);
+    return ADS_OK;
+}
+
+template <typename T, typename S>
+int upload(ads_ctx* ctx, T** out, const std::vector<S>& host)
+{
+    std::vector<T> tmp(host.begin(), host.end());
+    int rc = dev_alloc(ctx, out, tmp.size());
+    if (rc) return rc;
+    ADS_HIP(ctx, hipMemcpy(*out, tmp.data(), sizeof(T) * tmp.size(), hipMemcpyHostToDevice));
+    return ADS_OK;
+}
+
+template <typename T>
+int upload_dir(ads_ctx* ctx, const ads::Direction& d, DevDir<T>& dd, ads::DirTables<T>& dt, ads::SweepTables<T>& st)
+{
+    int rc;
+    if ((rc = upload(ctx, &dd.first_dof, d.first_dof))) return rc;
+    if ((rc = upload(ctx, &dd.tab, d.tab))) return rc;
+    if ((rc = upload(ctx, &dd.jac, d.jac))) return rc;
+    if ((rc = upload(ctx, &dd.wts, d.wts))) return rc;
+    if ((rc = upload(ctx, &dd.pts, d.pts))) return rc;
+    std::vector<double> fwd, bwd;
+    std::vector<int> piv;
+    ads::sweep_tables(d, fwd, piv, bwd);
+    if ((rc = upload(ctx, &dd.fwd, fwd))) return rc;
+    if ((rc = upload(ctx, &dd.piv, piv))) return rc;
+    if ((rc = upload(ctx, &dd.bwd, bwd))) return rc;
+    dt.first_dof = dd.first_dof;
+    dt.tab = dd.tab;
+    dt.jac = dd.jac;
+    dt.wts = dd.wts;
+    dt.pts = dd.pts;
+    dt.nelem = d.nelem;
+    dt.dofs = d.dofs;
+    st.fwd = dd.fwd;
+    st.piv = dd.piv;
+    st.bwd = dd.bwd;
+    st.n = d.dofs;
+    return ADS_OK;
+}
+
+int ensure_ir_scratch(ads_ctx* ctx)
+{
+    int rc;
+    if (!ctx->f32a && (rc = dev_alloc(ctx, &ctx->f32a, ctx->D))) return rc;
+    if (!ctx->f32b && (rc = dev_alloc(ctx, &ctx->f32b, ctx->D))) return rc;
+    if (!ctx->x64 && (rc = dev_alloc(ctx, &ctx->x64, ctx->D))) return rc;
+    if (!ctx->w0 && (rc = dev_alloc(ctx, &ctx->w0, ctx->D))) return rc;
+    if (!ctx->w1 && (rc = dev_alloc(ctx, &ctx->w1, ctx->D))) return rc;
+    if (!ctx->w2 && (rc = dev_alloc(ctx, &ctx->w2, 2 * ctx->D))) return rc;
+    return ADS_OK;
+}
+
+__global__ void k_fill_zero_d(double* p, long n)
+{
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+__global__ void k_fill_zero_f(float* p, long n)
+{
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+int zero(ads_ctx* ctx, double* p, long n)
+{
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n);
+    ADS_HIP(ctx, hipGetLastError());
+    return ADS_OK;
+}
+int zero(ads_ctx* ctx, float* p, long n)
+{
+    hipLaunchKernelGGL(k_fill_zero_f, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n);
+    ADS_HIP(ctx, hipGetLastError());
+    return ADS_OK;
+}
+
+// F = A(kappa) c (heat) or F = int f0 v (project), l2 -> l2_d
+template <typename T>
+int assemble(ads_ctx* ctx, int form, int f0, const double* c, T* F, double dt, const double* kappa, double* l2_d)
+{
+    int rc = zero(ctx, F, ctx->D);
+    if (rc) return rc;
+    int nparts = 0;
+    const ads::Tables<T>* tb;
+    if constexpr (sizeof(T) == 8) tb = &ctx->tb64;
+    else tb = &ctx->tb32;
+    ADS_HIP(ctx, ads::launch_assemble<T>(ctx->p, form, f0, c, F, *tb, dt, kappa, ctx->l2_part, &nparts, ctx->stream));
+    if (nparts > ctx->l2_cap) return fail(ctx, ADS_ERR_INVALID, "internal: l2 partial buffer too small");
+    ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, l2_d ? l2_d : ctx->l2, ctx->stream));
+    return ADS_OK;
+}
+
+int check_ctx(const ads_ctx* ctx) { return ctx ? ADS_OK : ADS_ERR_INVALID; }
+
+int read_l2(ads_ctx* ctx, double* l2norm)
+{
+    if (!l2norm) return ADS_OK;
+    ADS_HIP(ctx, hipMemcpyAsync(l2norm, ctx->l2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!std::isfinite(*l2norm)) return fail(ctx, ADS_ERR_NUMERIC, "l2norm is not finite");
+    return ADS_OK;
+}
+
+// c_out = M^-1 A c_in in FP32 + FP64-residual refinement (constant kappa).
+int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double kap, int refine, double* l2_d)
+{
+    int rc = ensure_ir_scratch(ctx);
+    if (rc) return rc;
+    ads::BandSet bs{};
+    for (int d = 0; d < 3; ++d) {
+        bs.mass[d] = ctx->mass[d];
+        bs.stiff[d] = ctx->stiff[d];
+        bs.n[d] = ctx->dir[d].dofs;
+    }
+    const double dtk = dt * kap;
+    // FP32 assembly (kappa folded into the constant) and FP32 sweeps
+    if ((rc = assemble<float>(ctx, ads::kFormHeat, 0, c_in, ctx->f32a, dtk, nullptr, l2_d))) return rc;
+    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32a, ctx->sw32, false, ctx->stream));
+    // first refinement: r = A c - M x0 (FP64), delta = M^-1 r (FP32)
+    ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, ctx->f32a, ctx->f32b, bs, dtk, ctx->w0,
+                                                          ctx->w1, ctx->w2, ctx->stream)));
+    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
+    ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, ctx->f32b, ctx->x64, ctx->D, ctx->stream));
+    for (int k = 1; k < refine; ++k) {
+        ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, ctx->x64, ctx->f32b, bs, dtk, ctx->w0,
+                                                               ctx->w1, ctx->w2, ctx->stream)));
+        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
+        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, ctx->x64, ctx->D, ctx->stream));
+    }
+    ADS_HIP(ctx, hipMemcpyAsync(c_out, ctx->x64, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    return ADS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ads_init(const int n[3], const int p[3], ads_ctx** out)
+{
+    if (!n || !p || !out) return fail(nullptr, ADS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (p[0] != p[1] || p[1] != p[2])
+        return fail(nullptr, ADS_ERR_UNSUPPORTED, "the GPU kernels need the same degree in x, y and z");
+    if (p[0] < 1 || p[0] > 4) return fail(nullptr, ADS_ERR_UNSUPPORTED, "degree must be 1..4 for the GPU kernels");
+    ads_ctx* ctx = new (std::nothrow) ads_ctx();
+    if (!ctx) return fail(nullptr, ADS_ERR_NOMEM, "host allocation failed");
+    ctx->p = p[0];
+    for (int d = 0; d < 3; ++d) {
+        std::string e = ads::build_direction(n[d], p[d], ctx->dir[d]);
+        if (!e.empty()) {
+            int code = e.find("singular") != std::string::npos ? ADS_ERR_SINGULAR : ADS_ERR_INVALID;
+            delete ctx;
+            return fail(nullptr, code, "direction " + std::to_string(d) + ": " + e);
+        }
+    }
+    ctx->D = static_cast<long>(ctx->dir[0].dofs) * ctx->dir[1].dofs * ctx->dir[2].dofs;
+    ctx->E = static_cast<long>(ctx->dir[0].nelem) * ctx->dir[1].nelem * ctx->dir[2].nelem;
+    if (ctx->D > (1L << 31) - 1) {
+        delete ctx;
+        return fail(nullptr, ADS_ERR_INVALID, "problem too large (more than 2^31 DOFs)");
+    }
+    int rc = ADS_OK;
+    hipError_t e = hipStreamCreate(&ctx->own);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, ADS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    ctx->stream = ctx->own;
+    for (int d = 0; d < 3 && !rc; ++d) {
+        rc = upload_dir(ctx, ctx->dir[d], ctx->d64[d], ctx->tb64.d[d], ctx->sw64[d]);
+        if (!rc) rc = upload_dir(ctx, ctx->dir[d], ctx->d32[d], ctx->tb32.d[d], ctx->sw32[d]);
+        if (!rc) rc = upload(ctx, &ctx->mass[d], ctx->dir[d].mass_band);
+        if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->dir[d].stiff_band);
+    }
+    if (!rc) rc = dev_alloc(ctx, &ctx->c, ctx->D);
+    if (!rc) rc = dev_alloc(ctx, &ctx->F, ctx->D);
+    ctx->l2_cap = ((ctx->dir[0].nelem + 63) / 64) * ctx->dir[1].nelem * ctx->dir[2].nelem;
+    if (!rc) rc = dev_alloc(ctx, &ctx->l2_part, ctx->l2_cap);
+    if (!rc) rc = dev_alloc(ctx, &ctx->l2, 1);
+    if (!rc) rc = zero(ctx, ctx->c, ctx->D);
+    if (!rc) rc = zero(ctx, ctx->l2, 1);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, ADS_ERR_HIP, "init sync failed");
+    if (rc) {
+        g_init_error = ctx->err.empty() ? g_init_error : ctx->err;
+        ads_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return ADS_OK;
+}
+
+int ads_destroy(ads_ctx* ctx)
+{
+    if (!ctx) return ADS_OK;
+    if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+    if (ctx->stream && ctx->stream != ctx->own) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return ADS_OK;
+}
+
+const char* ads_last_error(const ads_ctx* ctx) { return ctx ? ctx->err.c_str() : g_init_error.c_str(); }
+
+int ads_set_stream(ads_ctx* ctx, void* hip_stream)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own;
+    return ADS_OK;
+}
+
+int ads_synchronize(ads_ctx* ctx)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ADS_OK;
+}
+
+int ads_get_sizes(const ads_ctx* ctx, int dofs[3], int nelem[3], int p[3])
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    for (int d = 0; d < 3; ++d) {
+        if (dofs) dofs[d] = ctx->dir[d].dofs;
+        if (nelem) nelem[d] = ctx->dir[d].nelem;
+        if (p) p[d] = ctx->dir[d].p;
+    }
+    return ADS_OK;
+}
+
+int ads_get_knots(const ads_ctx* ctx, int dim, double* U)
+{
+    if (check_ctx(ctx) || dim < 0 || dim > 2 || !U) return ADS_ERR_INVALID;
+    std::memcpy(U, ctx->dir[dim].knots.data(), sizeof(double) * ctx->dir[dim].knots.size());
+    return ADS_OK;
+}
+
+int ads_get_basis(const ads_ctx* ctx, int dim, int* O, double* J, double* W, double* X, double* NN)
+{
+    if (check_ctx(ctx) || dim < 0 || dim > 2) return ADS_ERR_INVALID;
+    const ads::Direction& d = ctx->dir[dim];
+    if (O) std::memcpy(O, d.first_dof.data(), sizeof(int) * d.first_dof.size());
+    if (J) std::memcpy(J, d.jac.data(), sizeof(double) * d.jac.size());
+    if (W) std::memcpy(W, d.wts.data(), sizeof(double) * d.wts.size());
+    if (X) std::memcpy(X, d.pts.data(), sizeof(double) * d.pts.size());
+    if (NN) std::memcpy(NN, d.tab.data(), sizeof(double) * d.tab.size());
+    return ADS_OK;
+}
+
+int ads_get_matrix(const ads_ctx* ctx, int dim, const double mix[4], double* dense)
+{
+    if (check_ctx(ctx) || dim < 0 || dim > 2 || !mix || !dense) return ADS_ERR_INVALID;
+    ads::dense_matrix(ctx->dir[dim], mix, dense);
+    return ADS_OK;
+}
+
+int ads_set_coeffs(ads_ctx* ctx, const double* host)
+{
+    if (check_ctx(ctx) || !host) return ADS_ERR_INVALID;
+    ADS_HIP(ctx, hipMemcpyAsync(ctx->c, host, sizeof(double) * ctx->D, hipMemcpyHostToDevice, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ADS_OK;
+}
+
+int ads_get_coeffs(ads_ctx* ctx, double* host)
+{
+    if (check_ctx(ctx) || !host) return ADS_ERR_INVALID;
+    ADS_HIP(ctx, hipMemcpyAsync(host, ctx->c, sizeof(double) * ctx->D, hipMemcpyDeviceToHost, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ADS_OK;
+}
+
+int ads_coeffs_device(ads_ctx* ctx, double** dptr)
+{
+    if (check_ctx(ctx) || !dptr) return ADS_ERR_INVALID;
+    *dptr = ctx->c;
+    return ADS_OK;
+}
+
+int ads_set_kappa(ads_ctx* ctx, const double* host)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    if (!host) {
+        ctx->kappa = nullptr;
+        return ADS_OK;
+    }
+    for (long i = 0; i < ctx->E; ++i)
+        if (!std::isfinite(host[i])) return fail(ctx, ADS_ERR_NUMERIC, "kappa has a non-finite entry");
+    static_assert(sizeof(double) == 8, "");
+    if (!ctx->kappa) {
+        // keep the buffer across set/unset cycles
+        double* k = nullptr;
+        int rc = dev_alloc(ctx, &k, ctx->E);
+        if (rc) return rc;
+        ctx->kappa = k;
+    }
+    ADS_HIP(ctx, hipMemcpyAsync(ctx->kappa, host, sizeof(double) * ctx->E, hipMemcpyHostToDevice, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ADS_OK;
+}
+
+int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    if (f0_kind < ADS_F0_CONST || f0_kind > ADS_F0_BUMP3D) return fail(ctx, ADS_ERR_UNSUPPORTED, "unknown f0 kind");
+    int rc = assemble<double>(ctx, ads::kFormProject, f0_kind, nullptr, ctx->F, 0.0, nullptr, nullptr);
+    if (rc) return rc;
+    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, ctx->sw64, false, ctx->stream));
+    std::swap(ctx->c, ctx->F);
+    return read_l2(ctx, l2norm);
+}
+
+int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    int rc = ads_step_d(ctx, ctx->c, ctx->F, ctx->kappa, dt, prec, refine, nullptr);
+    if (rc) return rc;
+    std::swap(ctx->c, ctx->F);
+    return read_l2(ctx, l2norm);
+}
+
+int ads_last_l2norm(ads_ctx* ctx, double* l2norm)
+{
+    if (check_ctx(ctx) || !l2norm) return ADS_ERR_INVALID;
+    return read_l2(ctx, l2norm);
+}
+
+int ads_form_rhs_d(ads_ctx* ctx, const double* c_d, const double* kappa_d, double dt, double* F_d, double* l2_d)
+{
+    if (check_ctx(ctx) || !c_d || !F_d) return ADS_ERR_INVALID;
+    if (c_d == F_d) return fail(ctx, ADS_ERR_INVALID, "ads_form_rhs_d: c and F must not alias");
+    return assemble<double>(ctx, ads::kFormHeat, 0, c_d, F_d, dt, kappa_d, l2_d);
+}
+
+int ads_solve_d(ads_ctx* ctx, double* F_d, int trans)
+{
+    if (check_ctx(ctx) || !F_d) return ADS_ERR_INVALID;
+    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, F_d, ctx->sw64, trans != 0, ctx->stream));
+    return ADS_OK;
+}
+
+int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double* kappa_d, double dt, int prec,
+               int refine, double* l2_d)
+{
+    if (check_ctx(ctx) || !c_in_d || !c_out_d) return ADS_ERR_INVALID;
+    if (!std::isfinite(dt)) return fail(ctx, ADS_ERR_INVALID, "dt is not finite");
+    if (prec == ADS_PREC_FP64) {
+        double* target = (c_in_d == c_out_d) ? (c_out_d == ctx->F ? ctx->c : ctx->F) : c_out_d;
+        if (target == c_in_d) return fail(ctx, ADS_ERR_INVALID, "internal: buffer aliasing");
+        int rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in_d, target, dt, kappa_d, l2_d);
+        if (rc) return rc;
+        ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, target, ctx->sw64, false, ctx->stream));
+        if (target != c_out_d)
+            ADS_HIP(ctx, hipMemcpyAsync(c_out_d, target, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+        return ADS_OK;
+    }
+    if (prec == ADS_PREC_FP32_IR) {
+        if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
+        if (kappa_d) return fail(ctx, ADS_ERR_UNSUPPORTED,
+                                 "the FP32+refinement form needs a constant diffusivity (its FP64 residual is the "
+                                 "Kronecker banded operator)");
+        return step_ir(ctx, c_in_d, c_out_d, dt, 1.0, refine, l2_d);
+    }
+    return fail(ctx, ADS_ERR_INVALID, "unknown precision form");
+}
+
+int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next_d, const double* kappa_d, double dt,
+                       double* lam_out_d, double* dkappa_d)
+{
+    if (check_ctx(ctx) || !c_n_d || !lam_next_d || !lam_out_d) return ADS_ERR_INVALID;
+    int rc;
+    if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
+    // mu = M^-T lam_next
+    ADS_HIP(ctx, hipMemcpyAsync(ctx->mu, lam_next_d, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, true, ctx->stream));
+    // dkappa += -dt mu_e^T K_e c_n  (before lam_out may overwrite lam_next)
+    ADS_HIP(ctx, ads::launch_assemble_adjoint(ctx->p, c_n_d, ctx->mu, lam_out_d, ctx->tb64, dt, kappa_d, dkappa_d,
+                                              ctx->stream));
+    // lam_out = A(kappa) mu  (A symmetric)
+    return assemble<double>(ctx, ads::kFormHeat, 0, ctx->mu, lam_out_d, dt, kappa_d, nullptr);
+}
+
+int ads_host_basis(int n, int p, double* U, int* O, double* J, double* W, double* X, double* NN)
+{
+    ads::Direction d;
+    std::string e = ads::build_direction(n, p, d);
+    if (!e.empty()) return fail(nullptr, e.find("singular") != std::string::npos ? ADS_ERR_SINGULAR : ADS_ERR_INVALID, e);
+    if (U) std::memcpy(U, d.knots.data(), sizeof(double) * d.knots.size());
+    if (O) std::memcpy(O, d.first_dof.data(), sizeof(int) * d.first_dof.size());
+    if (J) std::memcpy(J, d.jac.data(), sizeof(double) * d.jac.size());
+    if (W) std::memcpy(W, d.wts.data(), sizeof(double) * d.wts.size());
+    if (X) std::memcpy(X, d.pts.data(), sizeof(double) * d.pts.size());
+    if (NN) std::memcpy(NN, d.tab.data(), sizeof(double) * d.tab.size());
+    return ADS_OK;
+}
+
+int ads_fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out)
+{
+    if (n < 0 || (n > 0 && !out)) return ADS_ERR_INVALID;
+    ads::fill_uniform(seed, n, lo, hi, out);
+    return ADS_OK;
+}
+
+int ads_fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double* out)
+{
+    if (n < 0 || (n > 0 && !out)) return ADS_ERR_INVALID;
+    ads::fill_lognormal(seed, n, mu, sigma, out);
+    return ADS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,305 @@
+// setup.cpp -- knots, quadrature, B-spline tables, 1-D matrices and banded
+// LU for one direction, on the host, once per run.
+//
+// Reference semantics followed (macwozni/ADS_MPI):
+//   knots         src/knot_vector.F90:79-92 (interior knots are REAL(4) quotients, :89)
+//   elements      src/basis.F90:91-97 (first DOF of each non-empty span)
+//   Gauss rule    src/gauss.F90:71-107 (closed forms for q <= 5)
+//   basis tables  src/basis.F90:99-110 with DersBasisFuns (:136-202), i.e.
+//                 Piegl & Tiller, The NURBS Book, algorithm A2.3
+//   matrices      src/projection_engine.F90:103-129 (MKBBT accumulation order)
+//   banded LU     LAPACK dgbtf2 (what dgbtrf runs for kl < its block size);
+//                 replaces the MUMPS factorisation of SolveOneDirection
+//                 (src/ADS.F90:352-361), including its zero-pivot failure.
+#include "setup.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace ads {
+
+namespace {
+
+constexpr int kMaxDegree = 9;
+
+// Gauss-Legendre nodes and weights on [-1, 1].
+bool gauss_rule(int q, std::vector<double>& x, std::vector<double>& w)
+{
+    x.assign(q, 0.0);
+    w.assign(q, 0.0);
+    switch (q) {
+    case 1:
+        x[0] = 0.0;
+        w[0] = 2.0;
+        return true;
+    case 2:
+        x[0] = -1.0 / std::sqrt(3.0);
+        x[1] = -x[0];
+        w[0] = w[1] = 1.0;
+        return true;
+    case 3:
+        x[0] = -std::sqrt(3.0 / 5.0);
+        x[1] = 0.0;
+        x[2] = -x[0];
+        w[0] = w[2] = 5.0 / 9.0;
+        w[1] = 8.0 / 9.0;
+        return true;
+    case 4: {
+        const double r = 2.0 * std::sqrt(6.0 / 5.0);
+        x[0] = -std::sqrt((3.0 + r) / 7.0);
+        x[1] = -std::sqrt((3.0 - r) / 7.0);
+        x[2] = -x[1];
+        x[3] = -x[0];
+        w[0] = w[3] = (18.0 - std::sqrt(30.0)) / 36.0;
+        w[1] = w[2] = (18.0 + std::sqrt(30.0)) / 36.0;
+        return true;
+    }
+    case 5: {
+        const double r = 2.0 * std::sqrt(10.0 / 7.0);
+        x[0] = -1.0 / 3.0 * std::sqrt(5.0 + r);
+        x[1] = -1.0 / 3.0 * std::sqrt(5.0 - r);
+        x[2] = 0.0;
+        x[3] = -x[1];
+        x[4] = -x[0];
+        w[0] = w[4] = (322.0 - 13.0 * std::sqrt(70.0)) / 900.0;
+        w[1] = w[3] = (322.0 + 13.0 * std::sqrt(70.0)) / 900.0;
+        w[2] = 128.0 / 225.0;
+        return true;
+    }
+    default:
+        break;
+    }
+    if (q < 1 || q > 10) return false;
+    // Newton on the Legendre polynomial for the remaining orders.
+    for (int i = 0; i < (q + 1) / 2; ++i) {
+        double z = -std::cos(M_PI * (i + 0.75) / (q + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 64; ++it) {
+            double pm = 1.0, pc = z;
+            for (int k = 2; k <= q; ++k) {
+                double pn = ((2 * k - 1) * z * pc - (k - 1) * pm) / k;
+                pm = pc;
+                pc = pn;
+            }
+            dp = q * (z * pc - pm) / (z * z - 1.0);
+            double step = pc / dp;
+            z -= step;
+            if (std::fabs(step) < 1e-17) break;
+        }
+        x[i] = z;
+        x[q - 1 - i] = -z;
+        w[i] = w[q - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    if (q % 2) x[q / 2] = 0.0;
+    return true;
+}
+
+// Values and first derivatives of the p+1 B-splines that are non-zero on
+// span i at u (A2.3 with n = 1 derivative).  out[a*2 + k].
+void basis_and_derivative(int span, double u, int p, const double* U, double* out)
+{
+    double left[kMaxDegree + 1], right[kMaxDegree + 1];
+    double ndu[kMaxDegree + 1][kMaxDegree + 1];  // ndu[j][r]: lower = knot diffs, upper = values
+    ndu[0][0] = 1.0;
+    for (int j = 1; j <= p; ++j) {
+        left[j] = u - U[span + 1 - j];
+        right[j] = U[span + j] - u;
+        double carry = 0.0;
+        for (int r = 0; r < j; ++r) {
+            ndu[j][r] = right[r + 1] + left[j - r];
+            const double t = ndu[r][j - 1] / ndu[j][r];
+            ndu[r][j] = carry + right[r + 1] * t;
+            carry = left[j - r] * t;
+        }
+        ndu[j][j] = carry;
+    }
+    for (int r = 0; r <= p; ++r) {
+        out[r * 2 + 0] = ndu[r][p];
+        // first derivative (k = 1 of A2.3)
+        double a_prev0 = 1.0, a_cur0 = 0.0, a_curk = 0.0;
+        double der = 0.0;
+        const int rk = r - 1, pk = p - 1;
+        if (r >= 1) {
+            a_cur0 = a_prev0 / ndu[pk + 1][rk];
+            der = a_cur0 * ndu[rk][pk];
+        }
+        // j loop of A2.3 is empty for k = 1 (j1 = 1 > j2 = 0 or j2 = p - r <= 0)
+        if (r <= pk) {
+            a_curk = -a_prev0 / ndu[pk + 1][r];
+            der = der + a_curk * ndu[r][pk];
+        }
+        out[r * 2 + 1] = der * p;
+    }
+}
+
+}  // namespace
+
+std::string build_direction(int n, int p, Direction& d)
+{
+    if (p < 1 || p > kMaxDegree) return "degree must be 1.." + std::to_string(kMaxDegree);
+    if (n < p) return "n must be >= p (at least one element)";
+    d = Direction();
+    d.n = n;
+    d.p = p;
+    d.q = p + 1;
+    d.dofs = n + 1;
+
+    // knots (FillOpenKnot): zeros, then ones, then the REAL(4) interior quotients
+    d.knots.assign(n + p + 2, 0.0);
+    for (int k = n + 1; k <= n + p + 1; ++k) d.knots[k] = 1.0;
+    const float denom = static_cast<float>(n - p + 1);
+    for (int k = p + 1; k <= n; ++k) {
+        const float num = static_cast<float>(k - p);
+        const float v = num / denom;
+        d.knots[k] = static_cast<double>(v);
+    }
+
+    // elements: non-empty spans, first DOF = span - p
+    const int m = n + p + 1;
+    for (int i = p; i <= m - p && i + 1 <= m; ++i)
+        if (d.knots[i] != d.knots[i + 1]) d.first_dof.push_back(i - p);
+    d.nelem = static_cast<int>(d.first_dof.size());
+    if (d.nelem != n - p + 1) return "knot vector has repeated interior knots";
+
+    std::vector<double> xg;
+    if (!gauss_rule(d.q, xg, d.wts)) return "unsupported quadrature order";
+
+    const int q = d.q, P = p + 1;
+    d.jac.resize(d.nelem);
+    d.pts.resize(static_cast<size_t>(d.nelem) * q);
+    d.tab.resize(static_cast<size_t>(d.nelem) * q * P * 2);
+    double vals[(kMaxDegree + 1) * 2];
+    for (int e = 0; e < d.nelem; ++e) {
+        const int span = d.first_dof[e] + p;
+        const double J = (d.knots[span + 1] - d.knots[span]) / 2.0;
+        d.jac[e] = J;
+        for (int iq = 0; iq < q; ++iq) {
+            const double u = (xg[iq] + 1.0) * J + d.knots[span];
+            d.pts[static_cast<size_t>(e) * q + iq] = u;
+            basis_and_derivative(span, u, p, d.knots.data(), vals);
+            for (int a = 0; a < P; ++a)
+                for (int k = 0; k < 2; ++k)
+                    d.tab[((static_cast<size_t>(e) * q + iq) * P + a) * 2 + k] = vals[a * 2 + k];
+        }
+    }
+
+    // 1-D mass and stiffness matrices, band form (half-bandwidth p)
+    const int bw = 2 * p + 1;
+    d.mass_band.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
+    d.stiff_band.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
+    for (int e = 0; e < d.nelem; ++e)
+        for (int iq = 0; iq < q; ++iq)
+            for (int c = 0; c < P; ++c)
+                for (int dd = 0; dd < P; ++dd) {
+                    const double* t = &d.tab[(static_cast<size_t>(e) * q + iq) * P * 2];
+                    const int row = d.first_dof[e] + c, col = d.first_dof[e] + dd;
+                    const size_t at = static_cast<size_t>(col) * bw + (p + row - col);
+                    d.mass_band[at] += t[c * 2] * t[dd * 2] * d.jac[e] * d.wts[iq];
+                    d.stiff_band[at] += t[c * 2 + 1] * t[dd * 2 + 1] * d.jac[e] * d.wts[iq];
+                }
+
+    // banded LU with partial pivoting (dgbtf2), kl = ku = p, ldab = 3p+1
+    const int kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1, N = d.dofs;
+    d.lu.assign(static_cast<size_t>(N) * ldab, 0.0);
+    d.piv.assign(N, 0);
+    auto AB = [&](int r, int j) -> double& { return d.lu[static_cast<size_t>(j) * ldab + r]; };
+    for (int j = 0; j < N; ++j)
+        for (int i = std::max(0, j - ku); i <= std::min(N - 1, j + kl); ++i)
+            AB(kv + i - j, j) = d.mass_band[static_cast<size_t>(j) * bw + (p + i - j)];
+    int reach = 0;  // last column touched by row interchanges so far
+    for (int j = 0; j < N; ++j) {
+        const int below = std::min(kl, N - 1 - j);
+        int best = 0;
+        double bestv = std::fabs(AB(kv, j));
+        for (int t = 1; t <= below; ++t)
+            if (std::fabs(AB(kv + t, j)) > bestv) {
+                bestv = std::fabs(AB(kv + t, j));
+                best = t;
+            }
+        d.piv[j] = j + best;
+        if (AB(kv + best, j) == 0.0)
+            return "singular 1-D matrix (zero pivot in column " + std::to_string(j + 1) + ")";
+        reach = std::max(reach, std::min(j + ku + best, N - 1));
+        if (best != 0)
+            for (int c = 0; c <= reach - j; ++c) std::swap(AB(kv + best - c, j + c), AB(kv - c, j + c));
+        if (below > 0) {
+            const double inv = 1.0 / AB(kv, j);
+            for (int t = 1; t <= below; ++t) AB(kv + t, j) *= inv;
+            for (int c = 1; c <= reach - j; ++c) {
+                const double y = AB(kv - c, j + c);
+                if (y == 0.0) continue;
+                for (int t = 1; t <= below; ++t) AB(kv + t - c, j + c) -= AB(kv + t, j) * y;
+            }
+        }
+    }
+    return std::string();
+}
+
+void dense_matrix(const Direction& d, const double mix[4], double* out)
+{
+    const int N = d.dofs, P = d.p + 1, q = d.q;
+    std::memset(out, 0, sizeof(double) * static_cast<size_t>(N) * N);
+    for (int e = 0; e < d.nelem; ++e)
+        for (int iq = 0; iq < q; ++iq)
+            for (int c = 0; c < P; ++c)
+                for (int dd = 0; dd < P; ++dd) {
+                    const double* t = &d.tab[(static_cast<size_t>(e) * q + iq) * P * 2];
+                    const double jw = d.jac[e] * d.wts[iq];
+                    const double M = t[c * 2] * t[dd * 2] * jw;
+                    const double K = t[c * 2 + 1] * t[dd * 2 + 1] * jw;
+                    const double B = t[c * 2 + 1] * t[dd * 2] * jw;
+                    const double BT = t[c * 2] * t[dd * 2 + 1] * jw;
+                    const int row = d.first_dof[e] + c, col = d.first_dof[e] + dd;
+                    out[static_cast<size_t>(row) * N + col] += mix[0] * M + mix[1] * K + mix[2] * B + mix[3] * BT;
+                }
+}
+
+void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>& fwd_piv,
+                  std::vector<double>& bwd)
+{
+    const int N = d.dofs, p = d.p, kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1;
+    const int K = kl + ku;
+    fwd.assign(static_cast<size_t>(N) * p, 0.0);
+    fwd_piv.assign(N, 0);
+    bwd.assign(static_cast<size_t>(N) * (K + 1), 0.0);
+    for (int j = 0; j < N; ++j) {
+        fwd_piv[j] = d.piv[j] - j;
+        for (int t = 1; t <= p; ++t)
+            if (j + t < N) fwd[static_cast<size_t>(j) * p + t - 1] = d.lu[static_cast<size_t>(j) * ldab + kv + t];
+        bwd[static_cast<size_t>(j) * (K + 1)] = 1.0 / d.lu[static_cast<size_t>(j) * ldab + kv];
+        for (int t = 1; t <= K; ++t)
+            if (j - t >= 0) bwd[static_cast<size_t>(j) * (K + 1) + t] = d.lu[static_cast<size_t>(j) * ldab + kv - t];
+    }
+}
+
+uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+static inline double unit_from(uint64_t key)
+{
+    return static_cast<double>(splitmix64(key) >> 11) * 0x1.0p-53;
+}
+
+void fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out)
+{
+    const uint64_t base = seed * 0xD1B54A32D192ED03ULL;
+    for (int64_t i = 0; i < n; ++i) out[i] = lo + (hi - lo) * unit_from(base + static_cast<uint64_t>(i));
+}
+
+void fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double* out)
+{
+    const uint64_t base = seed * 0xD1B54A32D192ED03ULL + 0x8CB92BA72F3D8DD7ULL;
+    for (int64_t i = 0; i < n; ++i) {
+        const double a = unit_from(base + 2 * static_cast<uint64_t>(i));
+        const double b = unit_from(base + 2 * static_cast<uint64_t>(i) + 1);
+        const double z = std::sqrt(-2.0 * std::log(1.0 - a)) * std::cos(2.0 * M_PI * b);
+        out[i] = std::exp(mu + sigma * z);
+    }
+}
+
+}  // namespace ads

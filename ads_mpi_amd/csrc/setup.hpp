@@ -1,0 +1,56 @@
+// setup.hpp -- host-side construction of one direction's discretisation:
+// knots, Gauss rule, B-spline tables, 1-D matrices and their banded LU.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ads {
+
+// One direction of the tensor-product space (one third of ADS_setup,
+// src/Setup.F90:29-112, on one rank).
+struct Direction {
+    int n = 0;      // number of basis functions minus one
+    int p = 0;      // degree
+    int q = 0;      // quadrature points per element = p + 1
+    int nelem = 0;  // elements (non-empty knot spans)
+    int dofs = 0;   // n + 1
+
+    std::vector<double> knots;   // U(0:n+p+1)
+    std::vector<int> first_dof;  // O(e), 0-based
+    std::vector<double> jac;     // J(e)
+    std::vector<double> wts;     // W(q)
+    std::vector<double> pts;     // X(iq, e) at [e*q + iq]
+    std::vector<double> tab;     // NN(k, a, iq, e) at [((e*q + iq)*(p+1) + a)*2 + k]
+
+    // 1-D matrices in band form, row-major by column: band[j*(2p+1) + (p + i - j)] = A(i, j)
+    std::vector<double> mass_band;
+    std::vector<double> stiff_band;
+
+    // LU factors of the mass matrix (partial pivoting, LAPACK dgbtrf layout:
+    // ldab = 3p+1, AB(kl+ku+i-j, j) at lu[j*ldab + kl+ku+i-j]) and pivots (0-based rows).
+    std::vector<double> lu;
+    std::vector<int> piv;
+};
+
+// Builds the direction; returns an empty string or an error message.
+std::string build_direction(int n, int p, Direction& d);
+
+// Dense (n+1)x(n+1) row-major mix(1)M + mix(2)K + mix(3)B + mix(4)B^T.
+void dense_matrix(const Direction& d, const double mix[4], double* out);
+
+// Sweep tables derived from the LU factors (see sweep kernels):
+//   fwd[j*p + t-1]      = L(j+t, j) multiplier, t = 1..p (0 past the end)
+//   fwd_piv[j]          = ipiv(j) - j in 0..p
+//   bwd[j*(2p+1) + 0]   = 1 / U(j, j)
+//   bwd[j*(2p+1) + t]   = U(j-t, j), t = 1..2p (0 before the start)
+void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>& fwd_piv,
+                  std::vector<double>& bwd);
+
+// Synthetic input generator shared with the host API (SplitMix64 counters).
+uint64_t splitmix64(uint64_t x);
+void fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
+void fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double* out);
+
+}  // namespace ads

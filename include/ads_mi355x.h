@@ -1,0 +1,153 @@
+/*
+ * ads_mi355x.h -- C ABI of the MI355X-native IGA-ADS heat-step hot path.
+ *
+ * Plain extern "C": pointers and sizes only.  Every function returns 0 on
+ * success or one of the ADS_ERR_* codes; ads_last_error() gives the text.
+ * Nothing here calls exit()/stop: the reference's fatal stops become codes.
+ *
+ * Layouts follow the reference (macwozni/ADS_MPI):
+ *   - coefficients and right-hand sides: doubles, x fastest,
+ *       c[x + (nx+1)*(y + (ny+1)*z)]          (projection_engine.F90:230)
+ *   - per-element fields (kappa, dkappa): ex fastest, E = nex*ney*nez
+ *   - basis tables as BasisData returns them   (basis.F90:74-112)
+ *
+ * "_d" entry points take device pointers and run on the context's stream
+ * (ads_set_stream); they do not synchronise.  Host-pointer entry points copy
+ * and synchronise.  Calls on one context are not reentrant.
+ */
+#ifndef ADS_MI355X_H
+#define ADS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADS_OK 0
+#define ADS_ERR_INVALID 1     /* bad argument or size                           */
+#define ADS_ERR_HIP 2         /* HIP runtime failure (message has the HIP error) */
+#define ADS_ERR_NOMEM 3       /* device or host allocation failed               */
+#define ADS_ERR_SINGULAR 4    /* zero pivot in a 1-D factorisation (ADS.F90:357-361) */
+#define ADS_ERR_UNSUPPORTED 5 /* order/form outside the compiled kernels        */
+#define ADS_ERR_NUMERIC 6     /* NaN/Inf in a checked scalar                    */
+
+/* initial states for ads_project (integrand of the t = 0 step) */
+#define ADS_F0_CONST 0   /* f = 1: problems/L2Projection/RHS_eq.F90:66-70          */
+#define ADS_F0_SINPROD 1 /* f = sin(pi x) sin(pi y) sin(pi z)                      */
+#define ADS_F0_BUMP3D 2  /* f = 2 bump3d(0.05, 0.4, x, y, z): problems/heat/input_data.F90:68 */
+
+/* precision forms */
+#define ADS_PREC_FP64 0    /* FP64 assembly and sweeps                            */
+#define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 Kronecker residual  */
+
+typedef struct ads_ctx ads_ctx;
+
+/* ---- set-up and teardown ------------------------------------------------ */
+
+/* ADSS::initialize(n, p, ads, ads_data, mierr) -- src/ADS.F90:11-107.
+ * n = number of basis functions minus one per direction (the heat CLI SIZE),
+ * p = B-spline degree per direction (1..4, equal in all directions).
+ * Builds the knots (PrepareKnot, knot_vector.F90:43-61), the basis tables
+ * (BasisData, basis.F90:74-112), the 1-D mass and stiffness matrices
+ * (MKBBT, projection_engine.F90:63-132) and their FP64 banded LU
+ * (dgbtrf form: replaces the MUMPS analysis+factorisation in
+ * SolveOneDirection, ADS.F90:294-361), and allocates device state. */
+int ads_init(const int n[3], const int p[3], ads_ctx** out);
+
+/* ADSS::Cleanup -- src/ADS.F90:802-863. */
+int ads_destroy(ads_ctx* ctx);
+
+/* Text of the last error on this context (or of the last failed ads_init
+ * when ctx is NULL). */
+const char* ads_last_error(const ads_ctx* ctx);
+
+/* Use this HIP stream (hipStream_t passed as void*) for all device work;
+ * NULL restores the context's own stream. */
+int ads_set_stream(ads_ctx* ctx, void* hip_stream);
+int ads_synchronize(ads_ctx* ctx);
+
+/* ---- host mirrors of ADS_setup (src/Setup.F90:29-112) -------------------- */
+
+/* dofs[d] = n(d)+1, nelem[d] = ads%nelem(d) */
+int ads_get_sizes(const ads_ctx* ctx, int dofs[3], int nelem[3], int p[3]);
+/* ads%Ux/Uy/Uz: n+p+2 doubles */
+int ads_get_knots(const ads_ctx* ctx, int dim, double* U);
+/* ads%Ox (0-based first DOF per element), ads%Jx, ads%Wx, ads%Xx(q, nelem),
+ * ads%NNx(0:1, 0:p, q, nelem); any pointer may be NULL. */
+int ads_get_basis(const ads_ctx* ctx, int dim, int* O, double* J, double* W, double* X, double* NN);
+/* 1-D matrix A = mix1*M + mix2*K + mix3*B + mix4*BT as a dense row-major
+ * (n+1)x(n+1) array: ComputeMatrix/MKBBT (projection_engine.F90:531-544). */
+int ads_get_matrix(const ads_ctx* ctx, int dim, const double mix[4], double* dense);
+
+/* ---- state held by the context (ads_data%F / R(:,2,2,2)) ----------------- */
+
+int ads_set_coeffs(ads_ctx* ctx, const double* host);
+int ads_get_coeffs(ads_ctx* ctx, double* host);
+/* device pointer of the context's coefficient vector (valid until destroy) */
+int ads_coeffs_device(ads_ctx* ctx, double** dptr);
+/* per-element diffusivity (E doubles, ex fastest); NULL restores kappa = 1 */
+int ads_set_kappa(ads_ctx* ctx, const double* host);
+
+/* ---- the step ------------------------------------------------------------ */
+
+/* ADSS::Step at t = 0 with the problem's t == 0 integrand
+ * (problems/heat/RHS_eq.F90:82-86): c = M^-1 F(f0).  l2norm (may be NULL)
+ * receives sum J*W*f0^2 = ||f0||^2 by quadrature (RHS_eq.F90:85). */
+int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
+
+/* ADSS::Step(iter, RHS_fun, ...) at t > 0 (src/ADS.F90:437-461) with the heat
+ * integrand (problems/heat/RHS_eq.F90:69-81, f = 0, kq = kappa_e):
+ *   F_a = sum_e sum_q J W (v_a u - dt kappa_e grad u . grad v_a)
+ *   c  <- (Mz (x) My (x) Mx)^-1 F      (Sub_Step, ADS.F90:471-794)
+ * prec = ADS_PREC_FP64 or ADS_PREC_FP32_IR (refine = number of FP64-residual
+ * refinement sweeps, 1..4).  l2norm (may be NULL: no host sync) receives
+ * sum J W u_n^2 of the state before the step (RHS_eq.F90:81). */
+int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm);
+
+/* l2norm of the last ads_step/ads_project (synchronises) */
+int ads_last_l2norm(ads_ctx* ctx, double* l2norm);
+
+/* ---- device-buffer forms (caller-owned buffers, context stream) ---------- */
+
+/* Form3DRHS after FormUn (projection_engine.F90:154-284, :306-441), heat
+ * integrand: F = A(kappa) c with A = M - dt K(kappa).  kappa_d may be NULL
+ * (kappa = 1).  l2_d (may be NULL) receives one double. */
+int ads_form_rhs_d(ads_ctx* ctx, const double* c_d, const double* kappa_d, double dt,
+                   double* F_d, double* l2_d);
+/* The three 1-D solves of Sub_Step (ADS.F90:514-779; SolveOneDirection
+ * job 3 per column, :363-368): F <- M^-1 F in place.  trans = 1 solves with
+ * M^T (dgbtrs 'T' semantics) for the adjoint. */
+int ads_solve_d(ads_ctx* ctx, double* F_d, int trans);
+/* One heat step c_out = M^-1 A(kappa) c_in (c_in may equal c_out). */
+int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double* kappa_d,
+               double dt, int prec, int refine, double* l2_d);
+/* Adjoint of ads_step_d (no reference counterpart; the reference has no
+ * gradients).  Given c_n and lam_next = dL/dc_{n+1}:
+ *   mu = M^-T lam_next,  lam_out = A(kappa) mu = dL/dc_n,
+ *   dkappa[e] += -dt sum_{q in e} J W grad c_n . grad mu   (if dkappa_d != NULL).
+ * lam_out may equal lam_next. */
+int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next_d,
+                       const double* kappa_d, double dt, double* lam_out_d, double* dkappa_d);
+
+/* ---- host-only helpers (no GPU needed) ----------------------------------- */
+
+/* The tables ads_init builds for one direction, without a device: knots U
+ * (n+p+2), O (nelem), J (nelem), W (p+1), X ((p+1)*nelem), NN
+ * (2*(p+1)*(p+1)*nelem) in the layouts of ads_get_knots/ads_get_basis.
+ * nelem = n - p + 1.  Any pointer may be NULL. */
+int ads_host_basis(int n, int p, double* U, int* O, double* J, double* W, double* X, double* NN);
+
+/* ---- synthetic inputs (host) --------------------------------------------- */
+
+/* SplitMix64 counter generator: u_i = (splitmix64(seed*0xD1B54A32D192ED03 + i) >> 11) * 2^-53,
+ * out[i] = lo + (hi - lo) u_i. */
+int ads_fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
+/* lognormal(mu, sigma) by Box-Muller on the same generator (key offset
+ * 0x8CB92BA72F3D8DD7, u1 = u_{2i}, u2 = u_{2i+1}). */
+int ads_fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADS_MI355X_H */

@@ -55,6 +55,8 @@ def lib():
         L.orc_fill_open_knot.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.orc_count_spans.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.orc_count_spans.restype = ctypes.c_int
+        L.orc_prepare_knot.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _ip]
+        L.orc_prepare_knot.restype = ctypes.c_int
         L.orc_gauss_rule.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_gauss_rule.restype = ctypes.c_int
         L.orc_ders_basis_funs.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
@@ -133,6 +135,16 @@ def knots(n, p, use_ref=False):
     else:
         lib().orc_fill_open_knot(n, p, _d(U))
     return U
+
+
+def prepare_knot(n, p):
+    """PrepareKnot: (U, nelem)."""
+    U = np.zeros(max(n + p + 2, 1))
+    ne = ctypes.c_int(0)
+    rc = lib().orc_prepare_knot(n, p, _d(U), ctypes.byref(ne))
+    if rc != 0:
+        raise ValueError("prepare_knot failed")
+    return U, ne.value
 
 
 def count_spans(n, p, U, use_ref=False):
