@@ -28,6 +28,10 @@ namespace {
 
 constexpr int kWave = 64;
 
+#ifndef ADS_QY_UNROLL
+#define ADS_QY_UNROLL 1
+#endif
+
 template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v)
 {
@@ -69,9 +73,24 @@ __device__ __forceinline__ T f0_eval(int kind, T x, T y, T z)
 // next one (lanes of a wave address overlapping columns across groups)
 __device__ __forceinline__ void lds_order() { __asm__ __volatile__("" ::: "memory"); }
 
+struct AsmDims {
+    int nex, ney, nez;  // elements per direction
+    int sx, sy;         // DOFs along x and y
+    int zseg;           // element layers per wave
+    int f0;             // initial-state kind (projection form)
+};
+
+// Every table pointer is a separate __restrict__ argument so the compiler
+// may read the wave-uniform y/z tables with scalar loads into SGPRs.
 template <typename T, int P, int FORM>
-__global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c, T* __restrict__ F, Tables<T> tb,
-                                                    T ndt, const double* __restrict__ kappa, int f0, int zseg,
+__global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c, T* __restrict__ F,
+                                                    const T* __restrict__ tabx, const T* __restrict__ taby,
+                                                    const T* __restrict__ tabz, const T* __restrict__ jacx,
+                                                    const T* __restrict__ jacy, const T* __restrict__ jacz,
+                                                    const T* __restrict__ Wx, const T* __restrict__ Wy,
+                                                    const T* __restrict__ Wz, const T* __restrict__ ptsx,
+                                                    const T* __restrict__ ptsy, const T* __restrict__ ptsz,
+                                                    const double* __restrict__ kappa, T ndt, AsmDims dm,
                                                     double* __restrict__ l2_part)
 {
     constexpr int Q = P, KL = P - 1, W = kWave + KL;
@@ -82,22 +101,18 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
     const int ex0 = blockIdx.x * kWave;
     const int ex = ex0 + lane;
     const int ey = blockIdx.y;
-    const int nex = tb.d[0].nelem, ney = tb.d[1].nelem, nez = tb.d[2].nelem;
-    const int ez0 = blockIdx.z * zseg;
-    const int ez1 = min(ez0 + zseg, nez);
-    const bool active = ex < nex;
-    const int exc = active ? ex : nex - 1;
-    const int sx = tb.d[0].dofs, sy = tb.d[1].dofs;
-    const long plane = static_cast<long>(sx) * sy;
+    const int ez0 = blockIdx.z * dm.zseg;
+    const int ez1 = min(ez0 + dm.zseg, dm.nez);
+    const bool active = ex < dm.nex;
+    const int exc = active ? ex : dm.nex - 1;
+    const int sx = dm.sx;
+    const long plane = static_cast<long>(sx) * dm.sy;
     const int oy = ey;  // first DOF of element e is e (open knots, checked on the host)
     const int ncol = min(W, sx - ex0);
 
-    const T* __restrict__ Nx = tb.d[0].tab + static_cast<size_t>(exc) * Q * P * 2;
-    const T* __restrict__ Ny = tb.d[1].tab + static_cast<size_t>(ey) * Q * P * 2;
-    const T* __restrict__ Wx = tb.d[0].wts;
-    const T* __restrict__ Wy = tb.d[1].wts;
-    const T* __restrict__ Wz = tb.d[2].wts;
-    const T Jxy = tb.d[0].jac[exc] * tb.d[1].jac[ey];
+    const T* __restrict__ Nx = tabx + static_cast<size_t>(exc) * Q * P * 2;
+    const T* __restrict__ Ny = taby + static_cast<size_t>(ey) * Q * P * 2;
+    const T Jxy = jacx[exc] * jacy[ey];
 
     for (int i = lane; i < P * P * W; i += kWave) (&fs[0][0][0])[i] = T(0);
 
@@ -129,11 +144,12 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
             load_plane(ez + KL);
             __syncthreads();
         }
-        const T* __restrict__ Nz = tb.d[2].tab + static_cast<size_t>(ez) * Q * P * 2;
-        const T J = Jxy * tb.d[2].jac[ez];
+        const T* __restrict__ Nz = tabz + static_cast<size_t>(ez) * Q * P * 2;
+        const T J = Jxy * jacz[ez];
         T gk0 = T(0);
         if constexpr (FORM == kFormHeat) {
-            const T kap = kappa ? T(kappa[exc + static_cast<size_t>(nex) * (ey + static_cast<size_t>(ney) * ez)]) : T(1);
+            const T kap =
+                kappa ? T(kappa[exc + static_cast<size_t>(dm.nex) * (ey + static_cast<size_t>(dm.ney) * ez)]) : T(1);
             gk0 = ndt * kap * J;
         }
         const int s0 = ez % P;
@@ -149,7 +165,7 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                 n0x[a] = Nx[(qx * P + a) * 2];
                 n1x[a] = Nx[(qx * P + a) * 2 + 1];
             }
-            // x stage: s0 = N0x c, s1 = N1x c   [ay][az]
+            // x stage: sx0 = N0x c, sx1 = N1x c   [ay][az]
             T sx0[P][P], sx1[P][P];
             if constexpr (FORM == kFormHeat) {
 #pragma unroll
@@ -161,8 +177,8 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                         T a1 = n1x[0] * col[0];
 #pragma unroll
                         for (int ax = 1; ax < P; ++ax) {
-                            a0 += n0x[ax] * col[ax];
-                            a1 += n1x[ax] * col[ax];
+                            a0 = fma(n0x[ax], col[ax], a0);
+                            a1 = fma(n1x[ax], col[ax], a1);
                         }
                         sx0[ay][az] = a0;
                         sx1[ay][az] = a1;
@@ -173,22 +189,28 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
             for (int ay = 0; ay < P; ++ay)
 #pragma unroll
                 for (int az = 0; az < P; ++az) t0[ay][az] = t1[ay][az] = T(0);
-            const T xq = (FORM == kFormProject) ? tb.d[0].pts[static_cast<size_t>(exc) * Q + qx] : T(0);
+            const T xq = (FORM == kFormProject) ? ptsx[static_cast<size_t>(exc) * Q + qx] : T(0);
             const T wxq = Wx[qx];
-#pragma unroll
+#pragma unroll ADS_QY_UNROLL
             for (int qy = 0; qy < Q; ++qy) {
+                T ny0[P], ny1[P];
+#pragma unroll
+                for (int a = 0; a < P; ++a) {
+                    ny0[a] = Ny[(qy * P + a) * 2];
+                    ny1[a] = Ny[(qy * P + a) * 2 + 1];
+                }
                 T r00[P], r01[P], r10[P];
                 if constexpr (FORM == kFormHeat) {
 #pragma unroll
                     for (int az = 0; az < P; ++az) {
-                        T a = Ny[(qy * P) * 2] * sx0[0][az];
-                        T b = Ny[(qy * P) * 2 + 1] * sx0[0][az];
-                        T d = Ny[(qy * P) * 2] * sx1[0][az];
+                        T a = ny0[0] * sx0[0][az];
+                        T b = ny1[0] * sx0[0][az];
+                        T d = ny0[0] * sx1[0][az];
 #pragma unroll
                         for (int ay = 1; ay < P; ++ay) {
-                            a += Ny[(qy * P + ay) * 2] * sx0[ay][az];
-                            b += Ny[(qy * P + ay) * 2 + 1] * sx0[ay][az];
-                            d += Ny[(qy * P + ay) * 2] * sx1[ay][az];
+                            a = fma(ny0[ay], sx0[ay][az], a);
+                            b = fma(ny1[ay], sx0[ay][az], b);
+                            d = fma(ny0[ay], sx1[ay][az], d);
                         }
                         r00[az] = a;
                         r01[az] = b;
@@ -198,41 +220,47 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                 T h00[P], h01[P], h10[P];
 #pragma unroll
                 for (int az = 0; az < P; ++az) h00[az] = h01[az] = h10[az] = T(0);
-                const T yq = (FORM == kFormProject) ? tb.d[1].pts[static_cast<size_t>(ey) * Q + qy] : T(0);
+                const T yq = (FORM == kFormProject) ? ptsy[static_cast<size_t>(ey) * Q + qy] : T(0);
                 const T wxy = wxq * Wy[qy];
 #pragma unroll
                 for (int qz = 0; qz < Q; ++qz) {
+                    T nz0[P], nz1[P];
+#pragma unroll
+                    for (int a = 0; a < P; ++a) {
+                        nz0[a] = Nz[(qz * P + a) * 2];
+                        nz1[a] = Nz[(qz * P + a) * 2 + 1];
+                    }
                     const T wq = wxy * Wz[qz];
                     const T w = J * wq;
                     if constexpr (FORM == kFormHeat) {
-                        T u = Nz[(qz * P) * 2] * r00[0];
-                        T uz = Nz[(qz * P) * 2 + 1] * r00[0];
-                        T uy = Nz[(qz * P) * 2] * r01[0];
-                        T ux = Nz[(qz * P) * 2] * r10[0];
+                        T u = nz0[0] * r00[0];
+                        T uz = nz1[0] * r00[0];
+                        T uy = nz0[0] * r01[0];
+                        T ux = nz0[0] * r10[0];
 #pragma unroll
                         for (int az = 1; az < P; ++az) {
-                            u += Nz[(qz * P + az) * 2] * r00[az];
-                            uz += Nz[(qz * P + az) * 2 + 1] * r00[az];
-                            uy += Nz[(qz * P + az) * 2] * r01[az];
-                            ux += Nz[(qz * P + az) * 2] * r10[az];
+                            u = fma(nz0[az], r00[az], u);
+                            uz = fma(nz1[az], r00[az], uz);
+                            uy = fma(nz0[az], r01[az], uy);
+                            ux = fma(nz0[az], r10[az], ux);
                         }
                         const T g0 = w * u;
-                        l2 += g0 * u;
+                        l2 = fma(g0, u, l2);
                         const T gk = gk0 * wq;
                         const T gx = gk * ux, gy = gk * uy, gz = gk * uz;
 #pragma unroll
                         for (int az = 0; az < P; ++az) {
-                            h00[az] += Nz[(qz * P + az) * 2] * g0 + Nz[(qz * P + az) * 2 + 1] * gz;
-                            h01[az] += Nz[(qz * P + az) * 2] * gy;
-                            h10[az] += Nz[(qz * P + az) * 2] * gx;
+                            h00[az] = fma(nz0[az], g0, fma(nz1[az], gz, h00[az]));
+                            h01[az] = fma(nz0[az], gy, h01[az]);
+                            h10[az] = fma(nz0[az], gx, h10[az]);
                         }
                     } else {
-                        const T zq = tb.d[2].pts[static_cast<size_t>(ez) * Q + qz];
-                        const T fv = f0_eval<T>(f0, xq, yq, zq);
+                        const T zq = ptsz[static_cast<size_t>(ez) * Q + qz];
+                        const T fv = f0_eval<T>(dm.f0, xq, yq, zq);
                         const T g0 = w * fv;
-                        l2 += g0 * fv;
+                        l2 = fma(g0, fv, l2);
 #pragma unroll
-                        for (int az = 0; az < P; ++az) h00[az] += Nz[(qz * P + az) * 2] * g0;
+                        for (int az = 0; az < P; ++az) h00[az] = fma(nz0[az], g0, h00[az]);
                     }
                 }
 #pragma unroll
@@ -240,10 +268,10 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
 #pragma unroll
                     for (int az = 0; az < P; ++az) {
                         if constexpr (FORM == kFormHeat) {
-                            t0[ay][az] += Ny[(qy * P + ay) * 2] * h00[az] + Ny[(qy * P + ay) * 2 + 1] * h01[az];
-                            t1[ay][az] += Ny[(qy * P + ay) * 2] * h10[az];
+                            t0[ay][az] = fma(ny0[ay], h00[az], fma(ny1[ay], h01[az], t0[ay][az]));
+                            t1[ay][az] = fma(ny0[ay], h10[az], t1[ay][az]);
                         } else {
-                            t0[ay][az] += Ny[(qy * P + ay) * 2] * h00[az];
+                            t0[ay][az] = fma(ny0[ay], h00[az], t0[ay][az]);
                         }
                     }
             }
@@ -255,10 +283,11 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                     for (int ay = 0; ay < P; ++ay)
 #pragma unroll
                         for (int az = 0; az < P; ++az) {
-                            T v;
-                            if constexpr (FORM == kFormHeat) v = n0x[ax] * t0[ay][az] + n1x[ax] * t1[ay][az];
-                            else v = n0x[ax] * t0[ay][az];
-                            fs[slot[az]][ay][lane + ax] += v;
+                            T& dst = fs[slot[az]][ay][lane + ax];
+                            if constexpr (FORM == kFormHeat)
+                                dst = fma(n0x[ax], t0[ay][az], fma(n1x[ax], t1[ay][az], dst));
+                            else
+                                dst = fma(n0x[ax], t0[ay][az], dst);
                         }
                     lds_order();
                 }
@@ -396,11 +425,19 @@ hipError_t assemble_p(const double* c, T* F, const Tables<T>& tb, double dt, con
                       double* l2_part, int* nparts, hipStream_t s)
 {
     const int xchunks = (tb.d[0].nelem + kWave - 1) / kWave;
-    const int zseg = segment_length(xchunks, tb.d[1].nelem, tb.d[2].nelem);
-    dim3 grid(xchunks, tb.d[1].nelem, (tb.d[2].nelem + zseg - 1) / zseg);
+    AsmDims dm;
+    dm.nex = tb.d[0].nelem;
+    dm.ney = tb.d[1].nelem;
+    dm.nez = tb.d[2].nelem;
+    dm.sx = tb.d[0].dofs;
+    dm.sy = tb.d[1].dofs;
+    dm.zseg = segment_length(xchunks, dm.ney, dm.nez);
+    dm.f0 = f0;
+    dim3 grid(xchunks, dm.ney, (dm.nez + dm.zseg - 1) / dm.zseg);
     *nparts = static_cast<int>(grid.x * grid.y * grid.z);
-    hipLaunchKernelGGL((k_assemble<T, P, FORM>), grid, dim3(kWave), 0, s, c, F, tb, static_cast<T>(-dt), kappa, f0,
-                       zseg, l2_part);
+    hipLaunchKernelGGL((k_assemble<T, P, FORM>), grid, dim3(kWave), 0, s, c, F, tb.d[0].tab, tb.d[1].tab,
+                       tb.d[2].tab, tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts,
+                       tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, kappa, static_cast<T>(-dt), dm, l2_part);
     return hipGetLastError();
 }
 
