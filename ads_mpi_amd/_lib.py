@@ -28,6 +28,7 @@ ADS_F0_BUMP3D = 2
 
 ADS_PREC_FP64 = 0
 ADS_PREC_FP32_IR = 1
+ADS_PREC_FP32_IR_KRON = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -80,6 +81,7 @@ def load(path: str | None = None):
     L.ads_project.argtypes = [_vp, ctypes.c_int, _dp]
     L.ads_step.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
     L.ads_last_l2norm.argtypes = [_vp, _dp]
+    L.ads_form_rhs.argtypes = [_vp, ctypes.c_double, _dp, _dp]
     L.ads_form_rhs_d.argtypes = [_vp, _vp, _vp, ctypes.c_double, _vp, _vp]
     L.ads_solve_d.argtypes = [_vp, _vp, ctypes.c_int]
     L.ads_step_d.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]

@@ -31,6 +31,9 @@ constexpr int kWave = 64;
 #ifndef ADS_QY_UNROLL
 #define ADS_QY_UNROLL 1
 #endif
+#ifndef ADS_NY_PREFETCH
+#define ADS_NY_PREFETCH 0
+#endif
 
 template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v)
@@ -69,16 +72,24 @@ __device__ __forceinline__ T f0_eval(int kind, T x, T y, T z)
     return T(2.0 * fv);
 }
 
-// keeps the compiler from moving LDS accesses of one x-offset group past the
-// next one (lanes of a wave address overlapping columns across groups)
-__device__ __forceinline__ void lds_order() { __asm__ __volatile__("" ::: "memory"); }
-
 struct AsmDims {
     int nex, ney, nez;  // elements per direction
     int sx, sy;         // DOFs along x and y
     int zseg;           // element layers per wave
     int f0;             // initial-state kind (projection form)
+    int xchunks;        // waves along x
 };
+
+// MI355X hands consecutive workgroups to its 8 XCDs round robin.  Remapping
+// the 1-D block index gives XCD k a contiguous run of (x chunk, ey, z
+// segment) tiles, so the waves that add into the same right-hand-side rows
+// (neighbouring ey) share one L2.  The map is a bijection for any count.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds, s = b / kXcds;
+    return k * q + min(k, r) + s;
+}
 
 // Every table pointer is a separate __restrict__ argument so the compiler
 // may read the wave-uniform y/z tables with scalar loads into SGPRs.
@@ -98,10 +109,13 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
     __shared__ T fs[P][P][W];                           // right-hand-side planes, ring by z
 
     const int lane = threadIdx.x;
-    const int ex0 = blockIdx.x * kWave;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int xc = tile % dm.xchunks;
+    const int ey = (tile / dm.xchunks) % dm.ney;
+    const int seg = tile / (dm.xchunks * dm.ney);
+    const int ex0 = xc * kWave;
     const int ex = ex0 + lane;
-    const int ey = blockIdx.y;
-    const int ez0 = blockIdx.z * dm.zseg;
+    const int ez0 = seg * dm.zseg;
     const int ez1 = min(ez0 + dm.zseg, dm.nez);
     const bool active = ex < dm.nex;
     const int exc = active ? ex : dm.nex - 1;
@@ -191,14 +205,39 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                 for (int az = 0; az < P; ++az) t0[ay][az] = t1[ay][az] = T(0);
             const T xq = (FORM == kFormProject) ? ptsx[static_cast<size_t>(exc) * Q + qx] : T(0);
             const T wxq = Wx[qx];
+#if ADS_NY_PREFETCH
+            T nyn0[P], nyn1[P];
+#pragma unroll
+            for (int a = 0; a < P; ++a) {
+                nyn0[a] = Ny[a * 2];
+                nyn1[a] = Ny[a * 2 + 1];
+            }
+#endif
 #pragma unroll ADS_QY_UNROLL
             for (int qy = 0; qy < Q; ++qy) {
                 T ny0[P], ny1[P];
+#if ADS_NY_PREFETCH
+                // this iteration's y tables were fetched one iteration ahead
+#pragma unroll
+                for (int a = 0; a < P; ++a) {
+                    ny0[a] = nyn0[a];
+                    ny1[a] = nyn1[a];
+                }
+                {
+                    const int qn = qy + 1 < Q ? qy + 1 : qy;
+#pragma unroll
+                    for (int a = 0; a < P; ++a) {
+                        nyn0[a] = Ny[(qn * P + a) * 2];
+                        nyn1[a] = Ny[(qn * P + a) * 2 + 1];
+                    }
+                }
+#else
 #pragma unroll
                 for (int a = 0; a < P; ++a) {
                     ny0[a] = Ny[(qy * P + a) * 2];
                     ny1[a] = Ny[(qy * P + a) * 2 + 1];
                 }
+#endif
                 T r00[P], r01[P], r10[P];
                 if constexpr (FORM == kFormHeat) {
 #pragma unroll
@@ -275,7 +314,9 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                         }
                     }
             }
-            // transposed x stage into the right-hand-side ring, one x offset at a time
+            // transposed x stage into the right-hand-side ring: LDS atomic adds
+            // (lanes of one wave address overlapping columns for different ax;
+            // the LDS applies one wave's adds in program order)
             if (active) {
 #pragma unroll
                 for (int ax = 0; ax < P; ++ax) {
@@ -283,13 +324,13 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
                     for (int ay = 0; ay < P; ++ay)
 #pragma unroll
                         for (int az = 0; az < P; ++az) {
-                            T& dst = fs[slot[az]][ay][lane + ax];
-                            if constexpr (FORM == kFormHeat)
-                                dst = fma(n0x[ax], t0[ay][az], fma(n1x[ax], t1[ay][az], dst));
-                            else
-                                dst = fma(n0x[ax], t0[ay][az], dst);
+                            T v;
+                            if constexpr (FORM == kFormHeat) v = fma(n0x[ax], t0[ay][az], n1x[ax] * t1[ay][az]);
+                            else v = n0x[ax] * t0[ay][az];
+                            atomic_add(&fs[slot[az]][ay][lane + ax], v);
                         }
-                    lds_order();
+                    // keep the scheduler from computing every group's values up front
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
         }
@@ -301,7 +342,7 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
 
     double l2d = active ? double(l2) : 0.0;
     l2d = wave_sum(l2d);
-    if (lane == 0) l2_part[blockIdx.x + gridDim.x * (blockIdx.y + static_cast<size_t>(gridDim.y) * blockIdx.z)] = l2d;
+    if (lane == 0) l2_part[blockIdx.x] = l2d;
 }
 
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ part, int n,
@@ -433,8 +474,9 @@ hipError_t assemble_p(const double* c, T* F, const Tables<T>& tb, double dt, con
     dm.sy = tb.d[1].dofs;
     dm.zseg = segment_length(xchunks, dm.ney, dm.nez);
     dm.f0 = f0;
-    dim3 grid(xchunks, dm.ney, (dm.nez + dm.zseg - 1) / dm.zseg);
-    *nparts = static_cast<int>(grid.x * grid.y * grid.z);
+    dm.xchunks = xchunks;
+    dim3 grid(xchunks * dm.ney * ((dm.nez + dm.zseg - 1) / dm.zseg));
+    *nparts = static_cast<int>(grid.x);
     hipLaunchKernelGGL((k_assemble<T, P, FORM>), grid, dim3(kWave), 0, s, c, F, tb.d[0].tab, tb.d[1].tab,
                        tb.d[2].tab, tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts,
                        tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, kappa, static_cast<T>(-dt), dm, l2_part);

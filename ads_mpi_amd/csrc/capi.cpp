@@ -237,6 +237,42 @@ int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double k
     return ADS_OK;
 }
 
+// c_out = M^-1 A c_in from a zero predictor: each pass computes the FP64
+// Kronecker residual r = A c - M x and corrects x with FP32 sweeps.
+int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double kap, int refine, double* l2_d)
+{
+    int rc = ensure_ir_scratch(ctx);
+    if (rc) return rc;
+    ads::BandSet bs{};
+    for (int d = 0; d < 3; ++d) {
+        bs.mass[d] = ctx->mass[d];
+        bs.stiff[d] = ctx->stiff[d];
+        bs.n[d] = ctx->dir[d].dofs;
+    }
+    const double dtk = dt * kap;
+    if ((rc = zero(ctx, ctx->x64, ctx->D))) return rc;
+    for (int k = 0; k < refine; ++k) {
+        if (k == 0)
+            ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, nullptr, ctx->f32b, bs, dtk, ctx->w0,
+                                                                  ctx->w1, ctx->w2, ctx->stream)));
+        else
+            ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, ctx->x64, ctx->f32b, bs, dtk,
+                                                                   ctx->w0, ctx->w1, ctx->w2, ctx->stream)));
+        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
+        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, ctx->x64, ctx->D, ctx->stream));
+    }
+    if (l2_d) {
+        // l2norm = c^T M c (the Gauss integral of u^2): M c through the residual
+        // stages with dtk = 0 and x = 0, then a deterministic dot product
+        if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
+        ADS_HIP(ctx, (ads::launch_kron_residual<float, double>(ctx->p, c_in, nullptr, ctx->mu, bs, 0.0, ctx->w0,
+                                                               ctx->w1, ctx->w2, ctx->stream)));
+        ADS_HIP(ctx, ads::launch_dot(c_in, ctx->mu, ctx->D, ctx->l2_part, ctx->l2_cap, l2_d, ctx->stream));
+    }
+    ADS_HIP(ctx, hipMemcpyAsync(c_out, ctx->x64, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    return ADS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -275,8 +311,8 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
     for (int d = 0; d < 3 && !rc; ++d) {
         rc = upload_dir(ctx, ctx->dir[d], ctx->d64[d], ctx->tb64.d[d], ctx->sw64[d]);
         if (!rc) rc = upload_dir(ctx, ctx->dir[d], ctx->d32[d], ctx->tb32.d[d], ctx->sw32[d]);
-        if (!rc) rc = upload(ctx, &ctx->mass[d], ctx->dir[d].mass_band);
-        if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->dir[d].stiff_band);
+        if (!rc) rc = upload(ctx, &ctx->mass[d], ctx->dir[d].mass_rows);
+        if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->dir[d].stiff_rows);
     }
     if (!rc) rc = dev_alloc(ctx, &ctx->c, ctx->D);
     if (!rc) rc = dev_alloc(ctx, &ctx->F, ctx->D);
@@ -418,9 +454,20 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm)
 int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm)
 {
     if (check_ctx(ctx)) return ADS_ERR_INVALID;
-    int rc = ads_step_d(ctx, ctx->c, ctx->F, ctx->kappa, dt, prec, refine, nullptr);
+    double* l2_d = (prec == ADS_PREC_FP32_IR_KRON && !l2norm) ? nullptr : ctx->l2;
+    int rc = ads_step_d(ctx, ctx->c, ctx->F, ctx->kappa, dt, prec, refine, l2_d);
     if (rc) return rc;
     std::swap(ctx->c, ctx->F);
+    return read_l2(ctx, l2norm);
+}
+
+int ads_form_rhs(ads_ctx* ctx, double dt, double* F_host, double* l2norm)
+{
+    if (check_ctx(ctx) || !F_host) return ADS_ERR_INVALID;
+    int rc = assemble<double>(ctx, ads::kFormHeat, 0, ctx->c, ctx->F, dt, ctx->kappa, nullptr);
+    if (rc) return rc;
+    ADS_HIP(ctx, hipMemcpyAsync(F_host, ctx->F, sizeof(double) * ctx->D, hipMemcpyDeviceToHost, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return read_l2(ctx, l2norm);
 }
 
@@ -465,6 +512,12 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
                                  "the FP32+refinement form needs a constant diffusivity (its FP64 residual is the "
                                  "Kronecker banded operator)");
         return step_ir(ctx, c_in_d, c_out_d, dt, 1.0, refine, l2_d);
+    }
+    if (prec == ADS_PREC_FP32_IR_KRON) {
+        if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
+        if (kappa_d) return fail(ctx, ADS_ERR_UNSUPPORTED,
+                                 "the Kronecker refinement form needs a constant diffusivity");
+        return step_ir_kron(ctx, c_in_d, c_out_d, dt, 1.0, refine, l2_d);
     }
     return fail(ctx, ADS_ERR_INVALID, "unknown precision form");
 }

@@ -60,15 +60,20 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
 
 // r = A c - M x in FP64 with the 1-D banded matrices (constant kappa):
 // A = M - dt*kappa*K  where M = Mz(x)My(x)Mx, K = Mz(x)My(x)Kx + Mz(x)Ky(x)Mx + Kz(x)My(x)Mx.
-// x is read in precision TX; r is written in precision TR.
+// x is read in precision TX (x == nullptr means x = 0); r is written in
+// precision TR.
 struct BandSet {
-    const double* mass[3];   // [n][2p+1] band by column
-    const double* stiff[3];  // [n][2p+1]
+    const double* mass[3];   // row tables: [i][t] = M(i, i-p+t), t = 0..2p, zero outside
+    const double* stiff[3];  // [i][t] = K(i, i-p+t)
     int n[3];
 };
 template <typename TX, typename TR>
 hipError_t launch_kron_residual(int p, const double* c, const TX* x, TR* r, const BandSet& bs, double dtk,
                                 double* work0, double* work1, double* work2, hipStream_t s);
+
+// out[0] = sum_i a_i b_i (deterministic: per-block partials, then one block)
+hipError_t launch_dot(const double* a, const double* b, int64_t n, double* part, int part_cap, double* out,
+                      hipStream_t s);
 
 // elementwise helpers
 hipError_t launch_convert_d2f(const double* in, float* out, int64_t n, hipStream_t s);

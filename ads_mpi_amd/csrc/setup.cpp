@@ -198,6 +198,16 @@ std::string build_direction(int n, int p, Direction& d)
                     d.stiff_band[at] += t[c * 2 + 1] * t[dd * 2 + 1] * d.jac[e] * d.wts[iq];
                 }
 
+    d.mass_rows.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
+    d.stiff_rows.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
+    for (int i = 0; i < d.dofs; ++i)
+        for (int t = 0; t < bw; ++t) {
+            const int j = i - p + t;
+            if (j < 0 || j >= d.dofs) continue;
+            d.mass_rows[static_cast<size_t>(i) * bw + t] = d.mass_band[static_cast<size_t>(j) * bw + (p + i - j)];
+            d.stiff_rows[static_cast<size_t>(i) * bw + t] = d.stiff_band[static_cast<size_t>(j) * bw + (p + i - j)];
+        }
+
     // banded LU with partial pivoting (dgbtf2), kl = ku = p, ldab = 3p+1
     const int kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1, N = d.dofs;
     d.lu.assign(static_cast<size_t>(N) * ldab, 0.0);

@@ -27,6 +27,9 @@ struct Direction {
     // 1-D matrices in band form, row-major by column: band[j*(2p+1) + (p + i - j)] = A(i, j)
     std::vector<double> mass_band;
     std::vector<double> stiff_band;
+    // the same matrices by row: rows[i*(2p+1) + t] = A(i, i-p+t), zero outside
+    std::vector<double> mass_rows;
+    std::vector<double> stiff_rows;
 
     // LU factors of the mass matrix (partial pivoting, LAPACK dgbtrf layout:
     // ldab = 3p+1, AB(kl+ku+i-j, j) at lu[j*ldab + kl+ku+i-j]) and pivots (0-based rows).

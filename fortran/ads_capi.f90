@@ -1,0 +1,102 @@
+! ads_capi.f90 -- ISO_C_BINDING interfaces to the C ABI in include/ads_mi355x.h.
+module ads_capi
+   use iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: ADS_OK = 0, ADS_ERR_INVALID = 1, ADS_ERR_HIP = 2, ADS_ERR_NOMEM = 3, &
+      ADS_ERR_SINGULAR = 4, ADS_ERR_UNSUPPORTED = 5, ADS_ERR_NUMERIC = 6
+   integer(c_int), parameter :: ADS_F0_CONST = 0, ADS_F0_SINPROD = 1, ADS_F0_BUMP3D = 2
+   integer(c_int), parameter :: ADS_PREC_FP64 = 0, ADS_PREC_FP32_IR = 1, ADS_PREC_FP32_IR_KRON = 2
+
+   interface
+      integer(c_int) function ads_init(n, p, ctx) bind(C, name="ads_init")
+         import :: c_int, c_ptr
+         integer(c_int), intent(in) :: n(3), p(3)
+         type(c_ptr), intent(out) :: ctx
+      end function
+      integer(c_int) function ads_destroy(ctx) bind(C, name="ads_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      type(c_ptr) function ads_last_error(ctx) bind(C, name="ads_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function ads_get_sizes(ctx, dofs, nelem, p) bind(C, name="ads_get_sizes")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: dofs(3), nelem(3), p(3)
+      end function
+      integer(c_int) function ads_get_knots(ctx, dim, U) bind(C, name="ads_get_knots")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: dim
+         real(c_double), intent(out) :: U(*)
+      end function
+      integer(c_int) function ads_get_basis(ctx, dim, O, J, W, X, NN) bind(C, name="ads_get_basis")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: dim
+         integer(c_int), intent(out) :: O(*)
+         real(c_double), intent(out) :: J(*), W(*), X(*), NN(*)
+      end function
+      integer(c_int) function ads_get_matrix(ctx, dim, mix, dense) bind(C, name="ads_get_matrix")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: dim
+         real(c_double), intent(in) :: mix(4)
+         real(c_double), intent(out) :: dense(*)
+      end function
+      integer(c_int) function ads_set_coeffs(ctx, host) bind(C, name="ads_set_coeffs")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: host(*)
+      end function
+      integer(c_int) function ads_get_coeffs(ctx, host) bind(C, name="ads_get_coeffs")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: host(*)
+      end function
+      integer(c_int) function ads_set_kappa(ctx, host) bind(C, name="ads_set_kappa")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         type(c_ptr), value :: host
+      end function
+      integer(c_int) function ads_project(ctx, f0_kind, l2norm) bind(C, name="ads_project")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: f0_kind
+         real(c_double), intent(out) :: l2norm
+      end function
+      integer(c_int) function ads_step(ctx, dt, prec, refine, l2norm) bind(C, name="ads_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: dt
+         integer(c_int), value :: prec, refine
+         real(c_double), intent(out) :: l2norm
+      end function
+   end interface
+
+contains
+
+   ! text of ads_last_error as a Fortran string
+   function ads_error_text(ctx) result(msg)
+      type(c_ptr), intent(in) :: ctx
+      character(len=:), allocatable :: msg
+      character(kind=c_char), pointer :: chars(:)
+      type(c_ptr) :: p
+      integer :: n
+      p = ads_last_error(ctx)
+      msg = ''
+      if (.not. c_associated(p)) return
+      call c_f_pointer(p, chars, [4096])
+      n = 0
+      do while (n < 4096)
+         if (chars(n + 1) == c_null_char) exit
+         n = n + 1
+      end do
+      allocate(character(len=n) :: msg)
+      msg = transfer(chars(1:n), msg)
+   end function
+
+end module ads_capi

@@ -39,7 +39,8 @@ extern "C" {
 
 /* precision forms */
 #define ADS_PREC_FP64 0    /* FP64 assembly and sweeps                            */
-#define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 Kronecker residual  */
+#define ADS_PREC_FP32_IR 1 /* FP32 quadrature assembly and sweeps + FP64 Kronecker residual refinement */
+#define ADS_PREC_FP32_IR_KRON 2 /* FP32 sweeps on FP64 Kronecker residuals from a zero start (constant kappa) */
 
 typedef struct ads_ctx ads_ctx;
 
@@ -84,7 +85,8 @@ int ads_get_matrix(const ads_ctx* ctx, int dim, const double mix[4], double* den
 
 int ads_set_coeffs(ads_ctx* ctx, const double* host);
 int ads_get_coeffs(ads_ctx* ctx, double* host);
-/* device pointer of the context's coefficient vector (valid until destroy) */
+/* device pointer of the context's coefficient vector; the context alternates
+ * two buffers, so the pointer is valid until the next ads_step/ads_project */
 int ads_coeffs_device(ads_ctx* ctx, double** dptr);
 /* per-element diffusivity (E doubles, ex fastest); NULL restores kappa = 1 */
 int ads_set_kappa(ads_ctx* ctx, const double* host);
@@ -100,10 +102,16 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  * integrand (problems/heat/RHS_eq.F90:69-81, f = 0, kq = kappa_e):
  *   F_a = sum_e sum_q J W (v_a u - dt kappa_e grad u . grad v_a)
  *   c  <- (Mz (x) My (x) Mx)^-1 F      (Sub_Step, ADS.F90:471-794)
- * prec = ADS_PREC_FP64 or ADS_PREC_FP32_IR (refine = number of FP64-residual
- * refinement sweeps, 1..4).  l2norm (may be NULL: no host sync) receives
- * sum J W u_n^2 of the state before the step (RHS_eq.F90:81). */
+ * prec = ADS_PREC_FP64, ADS_PREC_FP32_IR or ADS_PREC_FP32_IR_KRON (refine =
+ * number of FP64-residual refinement passes, 1..4).  l2norm (may be NULL: no
+ * host sync) receives sum J W u_n^2 of the state before the step
+ * (RHS_eq.F90:81); the Kronecker form computes it only when l2norm != NULL. */
 int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm);
+
+/* Form3DRHS after FormUn (projection_engine.F90:154-284) on the context's
+ * coefficients and diffusivity, heat integrand at t > 0, without advancing:
+ * F (host, x fastest) = A(kappa) c; l2norm (may be NULL) = sum J W u^2. */
+int ads_form_rhs(ads_ctx* ctx, double dt, double* F_host, double* l2norm);
 
 /* l2norm of the last ads_step/ads_project (synchronises) */
 int ads_last_l2norm(ads_ctx* ctx, double* l2norm);

@@ -241,11 +241,33 @@ def test_fp32_refinement_form():
         rng = np.random.default_rng(2)
         c = rng.uniform(-1, 1, C.D)
         ref = C.step_d(c, dt, prec=0)
-        raw32 = None
-        for refine in (1, 2):
-            got = C.step_d(c, dt, prec=1, refine=refine)
+        for prec, refine in ((1, 1), (1, 2), (2, 2), (2, 3)):
+            got = C.step_d(c, dt, prec=prec, refine=refine)
             err = rel(got, ref)
-            say("refine", refine, "rel err", err)
+            say("prec", prec, "refine", refine, "rel err", err)
             assert err <= 1e-10
     finally:
         C.close()
+
+
+def test_kron_ir_l2norm_and_state_api():
+    """The Kronecker refinement form through the state API: same coefficients
+    as the FP64 step to 1e-10, and its l2norm (c^T M c) equals the assembly's."""
+    n, p, dt = 40, 3, 1e-7
+    ads, data = adss.initialize(n, p)
+    try:
+        say("case kron ir state api")
+        c = np.random.default_rng(4).uniform(-1, 1, data.D)
+        data.F = c
+        rhs64 = adss.HeatRHS(Dt=dt, prec=adss.ADS_PREC_FP64)
+        rhs64.t = dt
+        l2a = adss.Step(1, rhs64, ads, data)
+        ref = data.F
+        data.F = c
+        rhs = adss.HeatRHS(Dt=dt, prec=_lib.ADS_PREC_FP32_IR_KRON, refine=2)
+        rhs.t = dt
+        l2b = adss.Step(1, rhs, ads, data)
+        assert rel(data.F, ref) <= 1e-10
+        assert abs(l2a - l2b) <= 1e-12 * abs(l2a)
+    finally:
+        adss.Cleanup(ads, data)
