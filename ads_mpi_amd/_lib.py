@@ -30,6 +30,9 @@ ADS_PREC_FP64 = 0
 ADS_PREC_FP32_IR = 1
 ADS_PREC_FP32_IR_KRON = 2
 
+ADS_KERNEL_HEAT_FP64 = 0
+ADS_KERNEL_KRON_RESIDUAL = 1
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
@@ -86,6 +89,7 @@ def load(path: str | None = None):
     L.ads_solve_d.argtypes = [_vp, _vp, ctypes.c_int]
     L.ads_step_d.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]
     L.ads_step_adjoint_d.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]
+    L.ads_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
     L.ads_fill_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_fill_lognormal.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_host_basis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _ip, _dp, _dp, _dp, _dp]

@@ -1,23 +1,35 @@
 // assemble.hip -- gfx950 kernels of the RHS assembly (FormUn + Form3DRHS).
 //
-// k_assemble fuses FormUn (projection_engine.F90:306-441) and Form3DRHS
-// (projection_engine.F90:154-284) for the heat and projection integrands
-// (problems/heat/RHS_eq.F90:35-88, problems/L2Projection/RHS_eq.F90:34-71):
+// Heat form (problems/heat/RHS_eq.F90:74-81, with a per-element diffusivity
+// kappa).  The integrand J W (v u - dt kappa grad u . grad v) is bilinear in
+// (u, v), and J = Jx Jy Jz, W = Wx Wy Wz and kappa are constant on an
+// element, so the Gauss sum over the Q^3 points of an element (Form3DRHS,
+// projection_engine.F90:154-284, over the point values of FormUn, :306-441)
+// factorises per direction into the 1-D element matrices
+//   Me(a, b) = sum_q N_a N_b J W_q,   Ke(a, b) = sum_q N'_a N'_b J W_q
+// (built once on the host from the same basis values):
+//   F_e  = (Mz(x)My(x)Mx - dt kappa (Mz(x)My(x)Kx + Mz(x)Ky(x)Mx + Kz(x)My(x)Mx)) c_e
+//   l2_e = c_e^T (Mz(x)My(x)Mx) c_e          (= sum over the points of J W u^2)
+// k_heat applies this operator one direction at a time: 8 P^4 multiply-adds
+// per element, against 18 P^4 for values and gradients at the points followed
+// by the transposed contractions.
+//   * a workgroup of four waves owns 64 consecutive elements along x, four
+//     consecutive y rows (one wave each) and a segment of z; one lane per
+//     element;
+//   * the coefficient planes z = ez..ez+p the current element layer touches
+//     sit in an LDS ring shared by the four waves ((64+p) x (4+p) values per
+//     plane), so a coefficient is read from HBM about (4+p)/4 times;
+//   * contributions accumulate in an LDS ring of right-hand-side planes by LDS
+//     atomic adds; a plane is complete once the element layer below it is
+//     done and is then added to F with coalesced global atomics (neighbouring
+//     workgroups in x, y and z share its edges).
 //
-//   * a wave owns 64 consecutive elements along x at one ey and walks a
-//     segment of ez; one lane per element;
-//   * the coefficients of the P planes z = ez..ez+p the current element layer
-//     touches sit in an LDS ring ("slab", (64+p) x P x P values), so each
-//     coefficient is fetched from HBM once per wave, coalesced along x;
-//   * values and gradients at the Q^3 Gauss points are sum-factorised one
-//     direction at a time (x stage: per-lane tables; y/z stages: wave-uniform
-//     tables in SGPRs), the integrand is applied pointwise and the transposed
-//     contractions accumulate into an LDS ring of right-hand-side planes;
-//   * a right-hand-side plane is complete once the element layer below it is
-//     done; it is then added to F with one coalesced atomic per entry (other
-//     waves own the neighbouring y rows and z segments).
+// Projection form (t = 0, RHS_eq.F90:82-86): f0 is sampled at the Gauss
+// points, so k_project keeps the point-wise quadrature, sum-factorised one
+// direction at a time.
 //
-// k_kappa_grad gives the adjoint's per-element diffusivity gradient.
+// The adjoint instance of k_heat (GRAD) also accumulates the per-element
+// diffusivity gradient.
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -27,13 +39,7 @@ namespace ads {
 namespace {
 
 constexpr int kWave = 64;
-
-#ifndef ADS_QY_UNROLL
-#define ADS_QY_UNROLL 1
-#endif
-#ifndef ADS_NY_PREFETCH
-#define ADS_NY_PREFETCH 0
-#endif
+constexpr int kRowsWg = 4;  // y rows (one wave each) per heat workgroup
 
 template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v)
@@ -46,6 +52,13 @@ __device__ __forceinline__ double wave_sum(double v)
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
+}
+
+// position of (a, b) in a packed upper triangle of a symmetric P x P matrix
+template <int P>
+__host__ __device__ constexpr int tri(int a, int b)
+{
+    return a <= b ? a * P - a * (a - 1) / 2 + (b - a) : b * P - b * (b - 1) / 2 + (a - b);
 }
 
 // initial states of the t = 0 projection
@@ -75,15 +88,16 @@ __device__ __forceinline__ T f0_eval(int kind, T x, T y, T z)
 struct AsmDims {
     int nex, ney, nez;  // elements per direction
     int sx, sy;         // DOFs along x and y
-    int zseg;           // element layers per wave
+    int zseg;           // element layers per workgroup
     int f0;             // initial-state kind (projection form)
-    int xchunks;        // waves along x
+    int xchunks;        // workgroups along x
+    int yblocks;        // workgroups along y
 };
 
 // MI355X hands consecutive workgroups to its 8 XCDs round robin.  Remapping
-// the 1-D block index gives XCD k a contiguous run of (x chunk, ey, z
-// segment) tiles, so the waves that add into the same right-hand-side rows
-// (neighbouring ey) share one L2.  The map is a bijection for any count.
+// the 1-D block index gives XCD k a contiguous run of (x chunk, y block, z
+// segment) tiles, so workgroups that add into the same right-hand-side rows
+// (neighbouring y blocks) share one L2.  The map is a bijection for any count.
 constexpr int kXcds = 8;
 __device__ __forceinline__ int xcd_tile(int b, int nb)
 {
@@ -92,21 +106,192 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 }
 
 // Every table pointer is a separate __restrict__ argument so the compiler
-// may read the wave-uniform y/z tables with scalar loads into SGPRs.
-template <typename T, int P, int FORM>
-__global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c, T* __restrict__ F,
-                                                    const T* __restrict__ tabx, const T* __restrict__ taby,
-                                                    const T* __restrict__ tabz, const T* __restrict__ jacx,
-                                                    const T* __restrict__ jacy, const T* __restrict__ jacz,
-                                                    const T* __restrict__ Wx, const T* __restrict__ Wy,
-                                                    const T* __restrict__ Wz, const T* __restrict__ ptsx,
-                                                    const T* __restrict__ ptsy, const T* __restrict__ ptsz,
-                                                    const double* __restrict__ kappa, T ndt, AsmDims dm,
-                                                    double* __restrict__ l2_part)
+// may read the wave-uniform y and z element matrices with scalar loads.
+// GRAD (adjoint, FP64): c is mu = M^-T lam and, in the same pass,
+// dkappa[e] += -dt cn_e^T K_e mu_e (K_e symmetric) from a third LDS ring
+// holding the forward state cn.
+template <typename T, int P, bool GRAD>
+__global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restrict__ c, T* __restrict__ F,
+                                                         const T* __restrict__ emx, const T* __restrict__ emy,
+                                                         const T* __restrict__ emz, const double* __restrict__ kappa,
+                                                         T ndt, AsmDims dm, double* __restrict__ l2_part,
+                                                         const double* __restrict__ cn, double* __restrict__ dkappa)
+{
+    constexpr int KL = P - 1, W = kWave + KL, R = kRowsWg + KL, NT = P * (P + 1) / 2;
+    constexpr int NTH = kWave * kRowsWg;
+    __shared__ T cs[P][R][W];  // coefficient planes, ring by z
+    __shared__ T fs[P][R][W];  // right-hand-side planes, ring by z
+    __shared__ T ns[GRAD ? P : 1][GRAD ? R : 1][GRAD ? W : 1];  // forward-state planes (GRAD)
+    __shared__ double l2w[kRowsWg];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int xc = tile % dm.xchunks;
+    const int yb = (tile / dm.xchunks) % dm.yblocks;
+    const int seg = tile / (dm.xchunks * dm.yblocks);
+    const int ex0 = xc * kWave, ey0 = yb * kRowsWg;
+    const int ex = ex0 + lane, ey = ey0 + wv;
+    const int ez0 = seg * dm.zseg;
+    const int ez1 = min(ez0 + dm.zseg, dm.nez);
+    const bool row_ok = ey < dm.ney;
+    const bool active = row_ok && ex < dm.nex;
+    const int exc = min(ex, dm.nex - 1), eyc = min(ey, dm.ney - 1);
+    const int sx = dm.sx;
+    const long plane = static_cast<long>(sx) * dm.sy;
+    const int ncol = min(W, sx - ex0);
+    const int nrow = min(R, dm.sy - ey0);
+
+    // first DOF of element e is e (open knots, checked on the host)
+    const T* __restrict__ Ex = emx + static_cast<size_t>(exc) * P * 2 * P;
+    const T* __restrict__ Ey = emy + static_cast<size_t>(eyc) * 2 * NT;
+
+    for (int i = tid; i < P * R * W; i += NTH) (&fs[0][0][0])[i] = T(0);
+
+    auto load_plane = [&](int z) {
+        const int slot = z % P;
+        const long off = ex0 + static_cast<long>(sx) * ey0 + plane * z;
+        for (int i = tid; i < R * W; i += NTH) {
+            const int r = i / W, col = i - r * W;
+            const bool in = col < ncol && r < nrow;
+            cs[slot][r][col] = in ? T(c[off + col + static_cast<long>(sx) * r]) : T(0);
+            if constexpr (GRAD) ns[slot][r][col] = in ? T(cn[off + col + static_cast<long>(sx) * r]) : T(0);
+        }
+    };
+    // each entry is always handled by the same thread, so consecutive
+    // flushes need no barrier between them
+    auto flush_plane = [&](int z) {
+        const int slot = z % P;
+        T* dst = F + ex0 + static_cast<long>(sx) * ey0 + plane * z;
+        for (int i = tid; i < R * W; i += NTH) {
+            const int r = i / W, col = i - r * W;
+            if (col < ncol && r < nrow) atomic_add(&dst[col + static_cast<long>(sx) * r], fs[slot][r][col]);
+            fs[slot][r][col] = T(0);
+        }
+    };
+
+    for (int z = ez0; z < ez0 + KL; ++z) load_plane(z);
+
+    T l2 = T(0);
+    for (int ez = ez0; ez < ez1; ++ez) {
+        // plane ez+KL goes to the slot of plane ez-1, whose readers finished
+        // before the previous iteration's second barrier
+        load_plane(ez + KL);
+        __syncthreads();
+        if (row_ok) {
+            const T* __restrict__ Ez = emz + static_cast<size_t>(ez) * 2 * NT;
+            const T kap =
+                kappa ? T(kappa[exc + static_cast<size_t>(dm.nex) * (eyc + static_cast<size_t>(dm.ney) * ez)]) : T(1);
+            const T s = ndt * kap;
+            int slot[P];
+#pragma unroll
+            for (int az = 0; az < P; ++az) slot[az] = (ez + az) % P;
+            T gacc = T(0);  // cn_e^T K_e c_e (GRAD)
+
+#pragma unroll 1
+            for (int axo = 0; axo < P; ++axo) {
+                // row axo of the lane's x element matrices; zero rows make an
+                // inactive lane add exact zeros
+                T mr[P], kr[P];
+#pragma unroll
+                for (int b = 0; b < P; ++b) {
+                    mr[b] = active ? Ex[(axo * 2) * P + b] : T(0);
+                    kr[b] = active ? Ex[(axo * 2 + 1) * P + b] : T(0);
+                }
+                // x: mx = (Mx c)[axo], kx = (Kx c)[axo]   [ay][az]
+                T mx[P][P], kx[P][P];
+#pragma unroll
+                for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+                    for (int az = 0; az < P; ++az) {
+                        const T* col = &cs[slot[az]][wv + ay][lane];
+                        T a = mr[0] * col[0];
+                        T k = kr[0] * col[0];
+#pragma unroll
+                        for (int b = 1; b < P; ++b) {
+                            a = fma(mr[b], col[b], a);
+                            k = fma(kr[b], col[b], k);
+                        }
+                        mx[ay][az] = a;
+                        kx[ay][az] = k;
+                    }
+#pragma unroll
+                for (int azo = 0; azo < P; ++azo) {
+                    // z: zm = Mz mx, zk = Mz kx + Kz mx   (output plane azo)   [ay]
+                    T zm[P], zk[P];
+#pragma unroll
+                    for (int ay = 0; ay < P; ++ay) {
+                        T m = Ez[tri<P>(azo, 0)] * mx[ay][0];
+                        T k = fma(Ez[NT + tri<P>(azo, 0)], mx[ay][0], Ez[tri<P>(azo, 0)] * kx[ay][0]);
+#pragma unroll
+                        for (int az = 1; az < P; ++az) {
+                            m = fma(Ez[tri<P>(azo, az)], mx[ay][az], m);
+                            k = fma(Ez[tri<P>(azo, az)], kx[ay][az], k);
+                            k = fma(Ez[NT + tri<P>(azo, az)], mx[ay][az], k);
+                        }
+                        zm[ay] = m;
+                        zk[ay] = k;
+                    }
+                    // y: M c = My zm, K c = My zk + Ky zm   (output row ayo)
+#pragma unroll
+                    for (int ayo = 0; ayo < P; ++ayo) {
+                        T om = Ey[tri<P>(ayo, 0)] * zm[0];
+                        T ok = fma(Ey[NT + tri<P>(ayo, 0)], zm[0], Ey[tri<P>(ayo, 0)] * zk[0]);
+#pragma unroll
+                        for (int ay = 1; ay < P; ++ay) {
+                            om = fma(Ey[tri<P>(ayo, ay)], zm[ay], om);
+                            ok = fma(Ey[tri<P>(ayo, ay)], zk[ay], ok);
+                            ok = fma(Ey[NT + tri<P>(ayo, ay)], zm[ay], ok);
+                        }
+                        l2 = fma(cs[slot[azo]][wv + ayo][lane + axo], om, l2);
+                        if constexpr (GRAD) gacc = fma(ns[slot[azo]][wv + ayo][lane + axo], ok, gacc);
+                        // lanes of one wave address distinct columns here; the
+                        // LDS applies one wave's adds in program order
+                        atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], fma(s, ok, om));
+                    }
+                    // keep the scheduler from computing every plane's values up front
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            if constexpr (GRAD)
+                if (active)
+                    dkappa[exc + static_cast<size_t>(dm.nex) * (eyc + static_cast<size_t>(dm.ney) * ez)] +=
+                        double(ndt * gacc);
+        }
+        __syncthreads();
+        flush_plane(ez);
+    }
+    // the last KL planes (their contributions finished before the loop's last barrier)
+    for (int z = ez1; z < ez1 + KL; ++z) flush_plane(z);
+
+    const double l2d = wave_sum(double(l2));
+    if (lane == 0) l2w[wv] = l2d;
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRowsWg; ++w) sum += l2w[w];
+        l2_part[blockIdx.x] = sum;
+    }
+}
+
+// t = 0 projection: F_a = sum_e sum_q J W f0(X_q) v_a(X_q), l2 = sum J W f0^2.
+// A wave owns 64 consecutive elements along x at one ey and walks a segment
+// of ez; f0 at the points is applied and the transposed contractions run
+// z, y, x into an LDS ring of right-hand-side planes (as in k_heat).
+template <typename T, int P>
+__global__ __launch_bounds__(kWave) void k_project(T* __restrict__ F, const T* __restrict__ tabx,
+                                                   const T* __restrict__ taby, const T* __restrict__ tabz,
+                                                   const T* __restrict__ jacx, const T* __restrict__ jacy,
+                                                   const T* __restrict__ jacz, const T* __restrict__ Wx,
+                                                   const T* __restrict__ Wy, const T* __restrict__ Wz,
+                                                   const T* __restrict__ ptsx, const T* __restrict__ ptsy,
+                                                   const T* __restrict__ ptsz, AsmDims dm,
+                                                   double* __restrict__ l2_part)
 {
     constexpr int Q = P, KL = P - 1, W = kWave + KL;
-    __shared__ T cs[FORM == kFormHeat ? P : 1][P][W];  // coefficient planes, ring by z
-    __shared__ T fs[P][P][W];                           // right-hand-side planes, ring by z
+    __shared__ T fs[P][P][W];  // right-hand-side planes, ring by z
 
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -121,7 +306,6 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
     const int exc = active ? ex : dm.nex - 1;
     const int sx = dm.sx;
     const long plane = static_cast<long>(sx) * dm.sy;
-    const int oy = ey;  // first DOF of element e is e (open knots, checked on the host)
     const int ncol = min(W, sx - ex0);
 
     const T* __restrict__ Nx = tabx + static_cast<size_t>(exc) * Q * P * 2;
@@ -129,43 +313,21 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
     const T Jxy = jacx[exc] * jacy[ey];
 
     for (int i = lane; i < P * P * W; i += kWave) (&fs[0][0][0])[i] = T(0);
-
-    auto load_plane = [&](int z) {
-        const int slot = z % P;
-        const double* src = c + ex0 + static_cast<long>(sx) * oy + plane * z;
-        for (int i = lane; i < P * W; i += kWave) {
-            const int ay = i / W, col = i - ay * W;
-            cs[slot][ay][col] = col < ncol ? T(src[col + static_cast<long>(sx) * ay]) : T(0);
-        }
-    };
     auto flush_plane = [&](int z) {
         const int slot = z % P;
-        T* dst = F + ex0 + static_cast<long>(sx) * oy + plane * z;
+        T* dst = F + ex0 + static_cast<long>(sx) * ey + plane * z;
         for (int i = lane; i < P * W; i += kWave) {
             const int ay = i / W, col = i - ay * W;
             if (col < ncol) atomic_add(&dst[col + static_cast<long>(sx) * ay], fs[slot][ay][col]);
             fs[slot][ay][col] = T(0);
         }
     };
-
-    if constexpr (FORM == kFormHeat)
-        for (int z = ez0; z < ez0 + KL; ++z) load_plane(z);
     __syncthreads();
 
     T l2 = T(0);
     for (int ez = ez0; ez < ez1; ++ez) {
-        if constexpr (FORM == kFormHeat) {
-            load_plane(ez + KL);
-            __syncthreads();
-        }
         const T* __restrict__ Nz = tabz + static_cast<size_t>(ez) * Q * P * 2;
         const T J = Jxy * jacz[ez];
-        T gk0 = T(0);
-        if constexpr (FORM == kFormHeat) {
-            const T kap =
-                kappa ? T(kappa[exc + static_cast<size_t>(dm.nex) * (ey + static_cast<size_t>(dm.ney) * ez)]) : T(1);
-            gk0 = ndt * kap * J;
-        }
         const int s0 = ez % P;
         int slot[P];
 #pragma unroll
@@ -173,165 +335,48 @@ __global__ __launch_bounds__(kWave) void k_assemble(const double* __restrict__ c
 
 #pragma unroll 1
         for (int qx = 0; qx < Q; ++qx) {
-            T n0x[P], n1x[P];
+            T n0x[P];
 #pragma unroll
-            for (int a = 0; a < P; ++a) {
-                n0x[a] = Nx[(qx * P + a) * 2];
-                n1x[a] = Nx[(qx * P + a) * 2 + 1];
-            }
-            // x stage: sx0 = N0x c, sx1 = N1x c   [ay][az]
-            T sx0[P][P], sx1[P][P];
-            if constexpr (FORM == kFormHeat) {
-#pragma unroll
-                for (int ay = 0; ay < P; ++ay)
-#pragma unroll
-                    for (int az = 0; az < P; ++az) {
-                        const T* col = &cs[slot[az]][ay][lane];
-                        T a0 = n0x[0] * col[0];
-                        T a1 = n1x[0] * col[0];
-#pragma unroll
-                        for (int ax = 1; ax < P; ++ax) {
-                            a0 = fma(n0x[ax], col[ax], a0);
-                            a1 = fma(n1x[ax], col[ax], a1);
-                        }
-                        sx0[ay][az] = a0;
-                        sx1[ay][az] = a1;
-                    }
-            }
-            T t0[P][P], t1[P][P];
+            for (int a = 0; a < P; ++a) n0x[a] = Nx[(qx * P + a) * 2];
+            T t0[P][P];
 #pragma unroll
             for (int ay = 0; ay < P; ++ay)
 #pragma unroll
-                for (int az = 0; az < P; ++az) t0[ay][az] = t1[ay][az] = T(0);
-            const T xq = (FORM == kFormProject) ? ptsx[static_cast<size_t>(exc) * Q + qx] : T(0);
+                for (int az = 0; az < P; ++az) t0[ay][az] = T(0);
+            const T xq = ptsx[static_cast<size_t>(exc) * Q + qx];
             const T wxq = Wx[qx];
-#if ADS_NY_PREFETCH
-            T nyn0[P], nyn1[P];
-#pragma unroll
-            for (int a = 0; a < P; ++a) {
-                nyn0[a] = Ny[a * 2];
-                nyn1[a] = Ny[a * 2 + 1];
-            }
-#endif
-#pragma unroll ADS_QY_UNROLL
+#pragma unroll 1
             for (int qy = 0; qy < Q; ++qy) {
-                T ny0[P], ny1[P];
-#if ADS_NY_PREFETCH
-                // this iteration's y tables were fetched one iteration ahead
+                T ny0[P];
 #pragma unroll
-                for (int a = 0; a < P; ++a) {
-                    ny0[a] = nyn0[a];
-                    ny1[a] = nyn1[a];
-                }
-                {
-                    const int qn = qy + 1 < Q ? qy + 1 : qy;
+                for (int a = 0; a < P; ++a) ny0[a] = Ny[(qy * P + a) * 2];
+                T h00[P];
 #pragma unroll
-                    for (int a = 0; a < P; ++a) {
-                        nyn0[a] = Ny[(qn * P + a) * 2];
-                        nyn1[a] = Ny[(qn * P + a) * 2 + 1];
-                    }
-                }
-#else
-#pragma unroll
-                for (int a = 0; a < P; ++a) {
-                    ny0[a] = Ny[(qy * P + a) * 2];
-                    ny1[a] = Ny[(qy * P + a) * 2 + 1];
-                }
-#endif
-                T r00[P], r01[P], r10[P];
-                if constexpr (FORM == kFormHeat) {
-#pragma unroll
-                    for (int az = 0; az < P; ++az) {
-                        T a = ny0[0] * sx0[0][az];
-                        T b = ny1[0] * sx0[0][az];
-                        T d = ny0[0] * sx1[0][az];
-#pragma unroll
-                        for (int ay = 1; ay < P; ++ay) {
-                            a = fma(ny0[ay], sx0[ay][az], a);
-                            b = fma(ny1[ay], sx0[ay][az], b);
-                            d = fma(ny0[ay], sx1[ay][az], d);
-                        }
-                        r00[az] = a;
-                        r01[az] = b;
-                        r10[az] = d;
-                    }
-                }
-                T h00[P], h01[P], h10[P];
-#pragma unroll
-                for (int az = 0; az < P; ++az) h00[az] = h01[az] = h10[az] = T(0);
-                const T yq = (FORM == kFormProject) ? ptsy[static_cast<size_t>(ey) * Q + qy] : T(0);
+                for (int az = 0; az < P; ++az) h00[az] = T(0);
+                const T yq = ptsy[static_cast<size_t>(ey) * Q + qy];
                 const T wxy = wxq * Wy[qy];
 #pragma unroll
                 for (int qz = 0; qz < Q; ++qz) {
-                    T nz0[P], nz1[P];
+                    const T w = J * (wxy * Wz[qz]);
+                    const T zq = ptsz[static_cast<size_t>(ez) * Q + qz];
+                    const T fv = f0_eval<T>(dm.f0, xq, yq, zq);
+                    const T g0 = w * fv;
+                    l2 = fma(g0, fv, l2);
 #pragma unroll
-                    for (int a = 0; a < P; ++a) {
-                        nz0[a] = Nz[(qz * P + a) * 2];
-                        nz1[a] = Nz[(qz * P + a) * 2 + 1];
-                    }
-                    const T wq = wxy * Wz[qz];
-                    const T w = J * wq;
-                    if constexpr (FORM == kFormHeat) {
-                        T u = nz0[0] * r00[0];
-                        T uz = nz1[0] * r00[0];
-                        T uy = nz0[0] * r01[0];
-                        T ux = nz0[0] * r10[0];
-#pragma unroll
-                        for (int az = 1; az < P; ++az) {
-                            u = fma(nz0[az], r00[az], u);
-                            uz = fma(nz1[az], r00[az], uz);
-                            uy = fma(nz0[az], r01[az], uy);
-                            ux = fma(nz0[az], r10[az], ux);
-                        }
-                        const T g0 = w * u;
-                        l2 = fma(g0, u, l2);
-                        const T gk = gk0 * wq;
-                        const T gx = gk * ux, gy = gk * uy, gz = gk * uz;
-#pragma unroll
-                        for (int az = 0; az < P; ++az) {
-                            h00[az] = fma(nz0[az], g0, fma(nz1[az], gz, h00[az]));
-                            h01[az] = fma(nz0[az], gy, h01[az]);
-                            h10[az] = fma(nz0[az], gx, h10[az]);
-                        }
-                    } else {
-                        const T zq = ptsz[static_cast<size_t>(ez) * Q + qz];
-                        const T fv = f0_eval<T>(dm.f0, xq, yq, zq);
-                        const T g0 = w * fv;
-                        l2 = fma(g0, fv, l2);
-#pragma unroll
-                        for (int az = 0; az < P; ++az) h00[az] = fma(nz0[az], g0, h00[az]);
-                    }
+                    for (int az = 0; az < P; ++az) h00[az] = fma(Nz[(qz * P + az) * 2], g0, h00[az]);
                 }
 #pragma unroll
                 for (int ay = 0; ay < P; ++ay)
 #pragma unroll
-                    for (int az = 0; az < P; ++az) {
-                        if constexpr (FORM == kFormHeat) {
-                            t0[ay][az] = fma(ny0[ay], h00[az], fma(ny1[ay], h01[az], t0[ay][az]));
-                            t1[ay][az] = fma(ny0[ay], h10[az], t1[ay][az]);
-                        } else {
-                            t0[ay][az] = fma(ny0[ay], h00[az], t0[ay][az]);
-                        }
-                    }
+                    for (int az = 0; az < P; ++az) t0[ay][az] = fma(ny0[ay], h00[az], t0[ay][az]);
             }
-            // transposed x stage into the right-hand-side ring: LDS atomic adds
-            // (lanes of one wave address overlapping columns for different ax;
-            // the LDS applies one wave's adds in program order)
             if (active) {
 #pragma unroll
-                for (int ax = 0; ax < P; ++ax) {
+                for (int ax = 0; ax < P; ++ax)
 #pragma unroll
                     for (int ay = 0; ay < P; ++ay)
 #pragma unroll
-                        for (int az = 0; az < P; ++az) {
-                            T v;
-                            if constexpr (FORM == kFormHeat) v = fma(n0x[ax], t0[ay][az], n1x[ax] * t1[ay][az]);
-                            else v = n0x[ax] * t0[ay][az];
-                            atomic_add(&fs[slot[az]][ay][lane + ax], v);
-                        }
-                    // keep the scheduler from computing every group's values up front
-                    __builtin_amdgcn_sched_barrier(0);
-                }
+                        for (int az = 0; az < P; ++az) atomic_add(&fs[slot[az]][ay][lane + ax], n0x[ax] * t0[ay][az]);
             }
         }
         __syncthreads();
@@ -360,157 +405,89 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// Per-element diffusivity gradient of the adjoint:
-//   dkappa[e] += -dt * mu_e^T K_e c_e,
-// K_e = Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz with the element's 1-D
-// matrices from the same basis tables and Gauss weights (exact for degree-2p
-// integrands), applied sum-factorised and dotted with mu.
-template <int P>
-__global__ __launch_bounds__(kWave) void k_kappa_grad(const double* __restrict__ c, const double* __restrict__ mu,
-                                                      Tables<double> tb, double ndt, double* __restrict__ dkappa)
+// z-segment length giving about `target` workgroups.
+int segment_length(long tiles_xy, int nez, long target)
 {
-    constexpr int Q = P;
-    const int lane = threadIdx.x;
-    const int ex = blockIdx.x * kWave + lane;
-    const int ey = blockIdx.y, ez = blockIdx.z;
-    const int nex = tb.d[0].nelem;
-    if (ex >= nex) return;
-    const int sx = tb.d[0].dofs, sy = tb.d[1].dofs;
-    const int ox = tb.d[0].first_dof[ex], oy = tb.d[1].first_dof[ey], oz = tb.d[2].first_dof[ez];
-    double Mx[P][P], Kx[P][P], My[P][P], Ky[P][P], Mz[P][P], Kz[P][P];
-    auto elem = [&](int dd, int e, double (*M)[P], double (*K)[P]) {
-        const double* N = tb.d[dd].tab + static_cast<size_t>(e) * Q * P * 2;
-        const double Jd = tb.d[dd].jac[e];
-#pragma unroll
-        for (int a = 0; a < P; ++a)
-#pragma unroll
-            for (int b = 0; b < P; ++b) {
-                double m = 0.0, k = 0.0;
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    const double jw = Jd * tb.d[dd].wts[q];
-                    m += N[(q * P + a) * 2] * N[(q * P + b) * 2] * jw;
-                    k += N[(q * P + a) * 2 + 1] * N[(q * P + b) * 2 + 1] * jw;
-                }
-                M[a][b] = m;
-                K[a][b] = k;
-            }
-    };
-    elem(0, ex, Mx, Kx);
-    elem(1, ey, My, Ky);
-    elem(2, ez, Mz, Kz);
-    const size_t base = static_cast<size_t>(ox) + static_cast<size_t>(sx) * (oy + static_cast<size_t>(sy) * oz);
-    double cc[P][P][P];
-#pragma unroll
-    for (int az = 0; az < P; ++az)
-#pragma unroll
-        for (int ay = 0; ay < P; ++ay)
-#pragma unroll
-            for (int ax = 0; ax < P; ++ax)
-                cc[ax][ay][az] = c[base + ax + static_cast<size_t>(sx) * (ay + static_cast<size_t>(sy) * az)];
-    double acc = 0.0;
-#pragma unroll 1
-    for (int a = 0; a < P; ++a) {
-        double kxc[P][P], mxc[P][P];
-#pragma unroll
-        for (int ay = 0; ay < P; ++ay)
-#pragma unroll
-            for (int az = 0; az < P; ++az) {
-                double s = 0.0, t = 0.0;
-#pragma unroll
-                for (int b = 0; b < P; ++b) {
-                    s += Kx[a][b] * cc[b][ay][az];
-                    t += Mx[a][b] * cc[b][ay][az];
-                }
-                kxc[ay][az] = s;
-                mxc[ay][az] = t;
-            }
-#pragma unroll
-        for (int by = 0; by < P; ++by) {
-            double u1[P], u2[P];
-#pragma unroll
-            for (int az = 0; az < P; ++az) {
-                double s = 0.0, t = 0.0;
-#pragma unroll
-                for (int ay = 0; ay < P; ++ay) {
-                    s += My[by][ay] * kxc[ay][az] + Ky[by][ay] * mxc[ay][az];
-                    t += My[by][ay] * mxc[ay][az];
-                }
-                u1[az] = s;
-                u2[az] = t;
-            }
-#pragma unroll
-            for (int bz = 0; bz < P; ++bz) {
-                double s = 0.0;
-#pragma unroll
-                for (int az = 0; az < P; ++az) s += Mz[bz][az] * u1[az] + Kz[bz][az] * u2[az];
-                acc += mu[base + a + static_cast<size_t>(sx) * (by + static_cast<size_t>(sy) * bz)] * s;
-            }
-        }
-    }
-    dkappa[ex + static_cast<size_t>(nex) * (ey + static_cast<size_t>(tb.d[1].nelem) * ez)] += ndt * acc;
-}
-
-// z-segment length: enough waves to fill 256 CUs about eight times over.
-int segment_length(int xchunks, int ney, int nez)
-{
-    const long target = 2048;
-    long nseg = (target + static_cast<long>(xchunks) * ney - 1) / (static_cast<long>(xchunks) * ney);
+    long nseg = (target + tiles_xy - 1) / tiles_xy;
     if (nseg < 1) nseg = 1;
     if (nseg > nez) nseg = nez;
     return static_cast<int>((nez + nseg - 1) / nseg);
 }
 
-template <typename T, int P, int FORM>
-hipError_t assemble_p(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, int f0,
-                      double* l2_part, int* nparts, hipStream_t s)
+template <typename T>
+AsmDims base_dims(const Tables<T>& tb)
 {
-    const int xchunks = (tb.d[0].nelem + kWave - 1) / kWave;
-    AsmDims dm;
+    AsmDims dm{};
     dm.nex = tb.d[0].nelem;
     dm.ney = tb.d[1].nelem;
     dm.nez = tb.d[2].nelem;
     dm.sx = tb.d[0].dofs;
     dm.sy = tb.d[1].dofs;
-    dm.zseg = segment_length(xchunks, dm.ney, dm.nez);
-    dm.f0 = f0;
-    dm.xchunks = xchunks;
-    dim3 grid(xchunks * dm.ney * ((dm.nez + dm.zseg - 1) / dm.zseg));
+    dm.xchunks = (dm.nex + kWave - 1) / kWave;
+    return dm;
+}
+
+template <typename T, int P, bool GRAD>
+hipError_t heat_p(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
+                  int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
+{
+    AsmDims dm = base_dims(tb);
+    dm.yblocks = (dm.ney + kRowsWg - 1) / kRowsWg;
+    dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.yblocks, dm.nez, 2048);
+    const int nseg = (dm.nez + dm.zseg - 1) / dm.zseg;
+    dim3 grid(dm.xchunks * dm.yblocks * nseg);
     *nparts = static_cast<int>(grid.x);
-    hipLaunchKernelGGL((k_assemble<T, P, FORM>), grid, dim3(kWave), 0, s, c, F, tb.d[0].tab, tb.d[1].tab,
-                       tb.d[2].tab, tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts,
-                       tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, kappa, static_cast<T>(-dt), dm, l2_part);
+    if (*nparts > part_cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_heat<T, P, GRAD>), grid, dim3(kWave * kRowsWg), 0, s, c, F, tb.d[0].emat,
+                       tb.d[1].emat_sym, tb.d[2].emat_sym, kappa, static_cast<T>(-dt), dm, l2_part, cn, dkappa);
     return hipGetLastError();
+}
+
+template <typename T, int P>
+hipError_t project_p(T* F, const Tables<T>& tb, int f0, double* l2_part, int part_cap, int* nparts, hipStream_t s)
+{
+    AsmDims dm = base_dims(tb);
+    dm.yblocks = dm.ney;
+    dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.ney, dm.nez, 2048);
+    dm.f0 = f0;
+    const int nseg = (dm.nez + dm.zseg - 1) / dm.zseg;
+    dim3 grid(dm.xchunks * dm.ney * nseg);
+    *nparts = static_cast<int>(grid.x);
+    if (*nparts > part_cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_project<T, P>), grid, dim3(kWave), 0, s, F, tb.d[0].tab, tb.d[1].tab, tb.d[2].tab,
+                       tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts, tb.d[0].pts,
+                       tb.d[1].pts, tb.d[2].pts, dm, l2_part);
+    return hipGetLastError();
+}
+
+template <typename T, int P>
+hipError_t assemble_p(int form, int f0, const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa,
+                      double* l2_part, int part_cap, int* nparts, hipStream_t s)
+{
+    return form == kFormHeat
+               ? heat_p<T, P, false>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, nullptr, nullptr, s)
+               : project_p<T, P>(F, tb, f0, l2_part, part_cap, nparts, s);
 }
 
 }  // namespace
 
 template <typename T>
 hipError_t launch_assemble(int p, int form, int f0, const double* c, T* F, const Tables<T>& tb, double dt,
-                           const double* kappa, double* l2_part, int* nparts, hipStream_t s)
+                           const double* kappa, double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
     switch (p) {
-    case 1:
-        return form == kFormHeat ? assemble_p<T, 2, kFormHeat>(c, F, tb, dt, kappa, f0, l2_part, nparts, s)
-                                 : assemble_p<T, 2, kFormProject>(c, F, tb, dt, kappa, f0, l2_part, nparts, s);
-    case 2:
-        return form == kFormHeat ? assemble_p<T, 3, kFormHeat>(c, F, tb, dt, kappa, f0, l2_part, nparts, s)
-                                 : assemble_p<T, 3, kFormProject>(c, F, tb, dt, kappa, f0, l2_part, nparts, s);
-    case 3:
-        return form == kFormHeat ? assemble_p<T, 4, kFormHeat>(c, F, tb, dt, kappa, f0, l2_part, nparts, s)
-                                 : assemble_p<T, 4, kFormProject>(c, F, tb, dt, kappa, f0, l2_part, nparts, s);
-    case 4:
-        return form == kFormHeat ? assemble_p<T, 5, kFormHeat>(c, F, tb, dt, kappa, f0, l2_part, nparts, s)
-                                 : assemble_p<T, 5, kFormProject>(c, F, tb, dt, kappa, f0, l2_part, nparts, s);
-    default:
-        return hipErrorInvalidValue;
+    case 1: return assemble_p<T, 2>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
+    case 2: return assemble_p<T, 3>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
+    case 3: return assemble_p<T, 4>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
+    case 4: return assemble_p<T, 5>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
+    default: return hipErrorInvalidValue;
     }
 }
 
 template hipError_t launch_assemble<double>(int, int, int, const double*, double*, const Tables<double>&, double,
-                                            const double*, double*, int*, hipStream_t);
+                                            const double*, double*, int, int*, hipStream_t);
 template hipError_t launch_assemble<float>(int, int, int, const double*, float*, const Tables<float>&, double,
-                                           const double*, double*, int*, hipStream_t);
+                                           const double*, double*, int, int*, hipStream_t);
 
 hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStream_t s)
 {
@@ -518,22 +495,19 @@ hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_adjoint(int p, const double* c_n, const double* mu, double* lam_out,
-                                   const Tables<double>& tb, double dt, const double* kappa, double* dkappa,
-                                   hipStream_t s)
+hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, double* lam_out, const Tables<double>& tb,
+                               double dt, const double* kappa, double* dkappa, double* l2_part, int part_cap,
+                               int* nparts, hipStream_t s)
 {
-    (void)lam_out;
-    (void)kappa;
-    if (!dkappa) return hipSuccess;
-    dim3 grid((tb.d[0].nelem + kWave - 1) / kWave, tb.d[1].nelem, tb.d[2].nelem);
+    if (!dkappa)
+        return launch_assemble<double>(p, kFormHeat, 0, mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, s);
     switch (p) {
-    case 1: hipLaunchKernelGGL((k_kappa_grad<2>), grid, dim3(kWave), 0, s, c_n, mu, tb, -dt, dkappa); break;
-    case 2: hipLaunchKernelGGL((k_kappa_grad<3>), grid, dim3(kWave), 0, s, c_n, mu, tb, -dt, dkappa); break;
-    case 3: hipLaunchKernelGGL((k_kappa_grad<4>), grid, dim3(kWave), 0, s, c_n, mu, tb, -dt, dkappa); break;
-    case 4: hipLaunchKernelGGL((k_kappa_grad<5>), grid, dim3(kWave), 0, s, c_n, mu, tb, -dt, dkappa); break;
+    case 1: return heat_p<double, 2, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 2: return heat_p<double, 3, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 3: return heat_p<double, 4, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 4: return heat_p<double, 5, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace ads

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -26,6 +27,8 @@ struct DevDir {
     T* jac = nullptr;
     T* wts = nullptr;
     T* pts = nullptr;
+    T* emat = nullptr;
+    T* emat_sym = nullptr;
     T* fwd = nullptr;
     int* piv = nullptr;
     T* bwd = nullptr;
@@ -62,9 +65,6 @@ struct ads_ctx {
     float* f32a = nullptr;
     float* f32b = nullptr;
     double* x64 = nullptr;
-    double* w0 = nullptr;
-    double* w1 = nullptr;
-    double* w2 = nullptr;  // 2*D
     double* mu = nullptr;
 
     std::vector<void*> allocs;
@@ -124,6 +124,8 @@ int upload_dir(ads_ctx* ctx, const ads::Direction& d, DevDir<T>& dd, ads::DirTab
     if ((rc = upload(ctx, &dd.jac, d.jac))) return rc;
     if ((rc = upload(ctx, &dd.wts, d.wts))) return rc;
     if ((rc = upload(ctx, &dd.pts, d.pts))) return rc;
+    if ((rc = upload(ctx, &dd.emat, d.emat))) return rc;
+    if ((rc = upload(ctx, &dd.emat_sym, d.emat_sym))) return rc;
     std::vector<double> fwd, bwd;
     std::vector<int> piv;
     ads::sweep_tables(d, fwd, piv, bwd);
@@ -135,6 +137,8 @@ int upload_dir(ads_ctx* ctx, const ads::Direction& d, DevDir<T>& dd, ads::DirTab
     dt.jac = dd.jac;
     dt.wts = dd.wts;
     dt.pts = dd.pts;
+    dt.emat = dd.emat;
+    dt.emat_sym = dd.emat_sym;
     dt.nelem = d.nelem;
     dt.dofs = d.dofs;
     st.fwd = dd.fwd;
@@ -150,9 +154,6 @@ int ensure_ir_scratch(ads_ctx* ctx)
     if (!ctx->f32a && (rc = dev_alloc(ctx, &ctx->f32a, ctx->D))) return rc;
     if (!ctx->f32b && (rc = dev_alloc(ctx, &ctx->f32b, ctx->D))) return rc;
     if (!ctx->x64 && (rc = dev_alloc(ctx, &ctx->x64, ctx->D))) return rc;
-    if (!ctx->w0 && (rc = dev_alloc(ctx, &ctx->w0, ctx->D))) return rc;
-    if (!ctx->w1 && (rc = dev_alloc(ctx, &ctx->w1, ctx->D))) return rc;
-    if (!ctx->w2 && (rc = dev_alloc(ctx, &ctx->w2, 2 * ctx->D))) return rc;
     return ADS_OK;
 }
 
@@ -190,8 +191,8 @@ int assemble(ads_ctx* ctx, int form, int f0, const double* c, T* F, double dt, c
     const ads::Tables<T>* tb;
     if constexpr (sizeof(T) == 8) tb = &ctx->tb64;
     else tb = &ctx->tb32;
-    ADS_HIP(ctx, ads::launch_assemble<T>(ctx->p, form, f0, c, F, *tb, dt, kappa, ctx->l2_part, &nparts, ctx->stream));
-    if (nparts > ctx->l2_cap) return fail(ctx, ADS_ERR_INVALID, "internal: l2 partial buffer too small");
+    ADS_HIP(ctx, ads::launch_assemble<T>(ctx->p, form, f0, c, F, *tb, dt, kappa, ctx->l2_part, ctx->l2_cap, &nparts,
+                                         ctx->stream));
     ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, l2_d ? l2_d : ctx->l2, ctx->stream));
     return ADS_OK;
 }
@@ -207,69 +208,76 @@ int read_l2(ads_ctx* ctx, double* l2norm)
     return ADS_OK;
 }
 
+ads::BandSet band_set(const ads_ctx* ctx)
+{
+    ads::BandSet bs{};
+    for (int d = 0; d < 3; ++d) {
+        bs.mass[d] = ctx->mass[d];
+        bs.stiff[d] = ctx->stiff[d];
+        bs.n[d] = ctx->dir[d].dofs;
+    }
+    return bs;
+}
+
 // c_out = M^-1 A c_in in FP32 + FP64-residual refinement (constant kappa).
 int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double kap, int refine, double* l2_d)
 {
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
-    ads::BandSet bs{};
-    for (int d = 0; d < 3; ++d) {
-        bs.mass[d] = ctx->mass[d];
-        bs.stiff[d] = ctx->stiff[d];
-        bs.n[d] = ctx->dir[d].dofs;
-    }
+    const ads::BandSet bs = band_set(ctx);
     const double dtk = dt * kap;
     // FP32 assembly (kappa folded into the constant) and FP32 sweeps
     if ((rc = assemble<float>(ctx, ads::kFormHeat, 0, c_in, ctx->f32a, dtk, nullptr, l2_d))) return rc;
     ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32a, ctx->sw32, false, ctx->stream));
-    // first refinement: r = A c - M x0 (FP64), delta = M^-1 r (FP32)
-    ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, ctx->f32a, ctx->f32b, bs, dtk, ctx->w0,
-                                                          ctx->w1, ctx->w2, ctx->stream)));
+    // first refinement: r = A c - M x0 (FP64), delta = M^-1 r (FP32); x
+    // accumulates in c_out unless the caller steps in place
+    double* x = c_out != c_in ? c_out : ctx->x64;
+    ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, ctx->f32a, ctx->f32b, bs, dtk, nullptr,
+                                                          ctx->stream)));
     ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-    ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, ctx->f32b, ctx->x64, ctx->D, ctx->stream));
+    ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, ctx->f32b, x, ctx->D, ctx->stream));
     for (int k = 1; k < refine; ++k) {
-        ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, ctx->x64, ctx->f32b, bs, dtk, ctx->w0,
-                                                               ctx->w1, ctx->w2, ctx->stream)));
+        ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, x, ctx->f32b, bs, dtk, nullptr,
+                                                               ctx->stream)));
         ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, ctx->x64, ctx->D, ctx->stream));
+        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, x, ctx->D, ctx->stream));
     }
-    ADS_HIP(ctx, hipMemcpyAsync(c_out, ctx->x64, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    if (x != c_out)
+        ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
     return ADS_OK;
 }
 
 // c_out = M^-1 A c_in from a zero predictor: each pass computes the FP64
-// Kronecker residual r = A c - M x and corrects x with FP32 sweeps.
+// Kronecker residual r = A c - M x and corrects x with FP32 sweeps.  The
+// first pass (x = 0) also gives l2norm = c^T M c, the Gauss integral of u^2.
 int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double kap, int refine, double* l2_d)
 {
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
-    ads::BandSet bs{};
-    for (int d = 0; d < 3; ++d) {
-        bs.mass[d] = ctx->mass[d];
-        bs.stiff[d] = ctx->stiff[d];
-        bs.n[d] = ctx->dir[d].dofs;
-    }
+    const ads::BandSet bs = band_set(ctx);
     const double dtk = dt * kap;
-    if ((rc = zero(ctx, ctx->x64, ctx->D))) return rc;
+    if (ads::kron_residual_partials(bs) > ctx->l2_cap)
+        return fail(ctx, ADS_ERR_INVALID, "internal: l2 partial buffer too small");
+    // x accumulates in c_out unless the caller steps in place
+    double* x = c_out != c_in ? c_out : ctx->x64;
     for (int k = 0; k < refine; ++k) {
-        if (k == 0)
-            ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, nullptr, ctx->f32b, bs, dtk, ctx->w0,
-                                                                  ctx->w1, ctx->w2, ctx->stream)));
-        else
-            ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, ctx->x64, ctx->f32b, bs, dtk,
-                                                                   ctx->w0, ctx->w1, ctx->w2, ctx->stream)));
+        if (k == 0) {
+            ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, nullptr, ctx->f32b, bs, dtk,
+                                                                  l2_d ? ctx->l2_part : nullptr, ctx->stream)));
+            if (l2_d)
+                ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, ads::kron_residual_partials(bs), l2_d,
+                                                         ctx->stream));
+        } else {
+            ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, x, ctx->f32b, bs, dtk, nullptr,
+                                                                   ctx->stream)));
+        }
         ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, ctx->x64, ctx->D, ctx->stream));
+        // x = delta on the first pass (no read of x), x += delta after
+        ADS_HIP(ctx, ads::launch_add_f2d(k == 0 ? ctx->f32b : nullptr, k == 0 ? nullptr : ctx->f32b, x, ctx->D,
+                                         ctx->stream));
     }
-    if (l2_d) {
-        // l2norm = c^T M c (the Gauss integral of u^2): M c through the residual
-        // stages with dtk = 0 and x = 0, then a deterministic dot product
-        if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
-        ADS_HIP(ctx, (ads::launch_kron_residual<float, double>(ctx->p, c_in, nullptr, ctx->mu, bs, 0.0, ctx->w0,
-                                                               ctx->w1, ctx->w2, ctx->stream)));
-        ADS_HIP(ctx, ads::launch_dot(c_in, ctx->mu, ctx->D, ctx->l2_part, ctx->l2_cap, l2_d, ctx->stream));
-    }
-    ADS_HIP(ctx, hipMemcpyAsync(c_out, ctx->x64, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    if (x != c_out)
+        ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
     return ADS_OK;
 }
 
@@ -316,7 +324,12 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
     }
     if (!rc) rc = dev_alloc(ctx, &ctx->c, ctx->D);
     if (!rc) rc = dev_alloc(ctx, &ctx->F, ctx->D);
-    ctx->l2_cap = ((ctx->dir[0].nelem + 63) / 64) * ctx->dir[1].nelem * ctx->dir[2].nelem;
+    {
+        // partial sums: at most one per (64 x elements, y row, z layer) in the
+        // assembly, and kron_residual_partials in the Kronecker residual
+        const long asm_cap = static_cast<long>((ctx->dir[0].nelem + 63) / 64) * ctx->dir[1].nelem * ctx->dir[2].nelem;
+        ctx->l2_cap = static_cast<int>(std::max<long>(asm_cap, ads::kron_residual_partials(band_set(ctx))));
+    }
     if (!rc) rc = dev_alloc(ctx, &ctx->l2_part, ctx->l2_cap);
     if (!rc) rc = dev_alloc(ctx, &ctx->l2, 1);
     if (!rc) rc = zero(ctx, ctx->c, ctx->D);
@@ -454,8 +467,7 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm)
 int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm)
 {
     if (check_ctx(ctx)) return ADS_ERR_INVALID;
-    double* l2_d = (prec == ADS_PREC_FP32_IR_KRON && !l2norm) ? nullptr : ctx->l2;
-    int rc = ads_step_d(ctx, ctx->c, ctx->F, ctx->kappa, dt, prec, refine, l2_d);
+    int rc = ads_step_d(ctx, ctx->c, ctx->F, ctx->kappa, dt, prec, refine, ctx->l2);
     if (rc) return rc;
     std::swap(ctx->c, ctx->F);
     return read_l2(ctx, l2norm);
@@ -526,16 +538,18 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
                        double* lam_out_d, double* dkappa_d)
 {
     if (check_ctx(ctx) || !c_n_d || !lam_next_d || !lam_out_d) return ADS_ERR_INVALID;
+    if (lam_out_d == c_n_d) return fail(ctx, ADS_ERR_INVALID, "ads_step_adjoint_d: lam_out must not alias c_n");
     int rc;
     if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
     // mu = M^-T lam_next
     ADS_HIP(ctx, hipMemcpyAsync(ctx->mu, lam_next_d, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
     ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, true, ctx->stream));
-    // dkappa += -dt mu_e^T K_e c_n  (before lam_out may overwrite lam_next)
-    ADS_HIP(ctx, ads::launch_assemble_adjoint(ctx->p, c_n_d, ctx->mu, lam_out_d, ctx->tb64, dt, kappa_d, dkappa_d,
-                                              ctx->stream));
-    // lam_out = A(kappa) mu  (A symmetric)
-    return assemble<double>(ctx, ads::kFormHeat, 0, ctx->mu, lam_out_d, dt, kappa_d, nullptr);
+    // lam_out = A(kappa) mu (A symmetric) and dkappa += -dt c_n,e^T K_e mu_e in one pass
+    if ((rc = zero(ctx, lam_out_d, ctx->D))) return rc;
+    int nparts = 0;
+    ADS_HIP(ctx, ads::launch_heat_adjoint(ctx->p, ctx->mu, c_n_d, lam_out_d, ctx->tb64, dt, kappa_d, dkappa_d,
+                                          ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream));
+    return ADS_OK;
 }
 
 int ads_host_basis(int n, int p, double* U, int* O, double* J, double* W, double* X, double* NN)
@@ -563,6 +577,44 @@ int ads_fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double
 {
     if (n < 0 || (n > 0 && !out)) return ADS_ERR_INVALID;
     ads::fill_lognormal(seed, n, mu, sigma, out);
+    return ADS_OK;
+}
+
+int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms_per_launch)
+{
+    if (check_ctx(ctx) || !ms_per_launch || launches < 1) return ADS_ERR_INVALID;
+    if (which != ADS_KERNEL_HEAT_FP64 && which != ADS_KERNEL_KRON_RESIDUAL)
+        return fail(ctx, ADS_ERR_INVALID, "ads_time_kernel: unknown kernel");
+    int rc = ensure_ir_scratch(ctx);
+    if (rc) return rc;
+    if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
+    const ads::BandSet bs = band_set(ctx);
+    if ((rc = zero(ctx, ctx->mu, ctx->D))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ADS_HIP(ctx, hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(ctx, e, "hipEventCreate");
+    }
+    e = hipEventRecord(e0, ctx->stream);
+    int nparts = 0;
+    for (int i = 0; i < launches && e == hipSuccess; ++i) {
+        if (which == ADS_KERNEL_HEAT_FP64)
+            e = ads::launch_assemble<double>(ctx->p, ads::kFormHeat, 0, ctx->c, ctx->mu, ctx->tb64, dt, ctx->kappa,
+                                             ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream);
+        else
+            e = ads::launch_kron_residual<float, float>(ctx->p, ctx->c, nullptr, ctx->f32b, bs, dt, ctx->l2_part,
+                                                        ctx->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(ctx, e, "ads_time_kernel");
+    *ms_per_launch = static_cast<double>(ms) / launches;
     return ADS_OK;
 }
 

@@ -15,6 +15,8 @@ struct DirTables {
     const T* jac;          // J(e)
     const T* wts;          // W(iq)
     const T* pts;          // X(iq, e) at [e*q + iq]
+    const T* emat;         // element matrices by rows [((e*P + a)*2 + k)*P + b]
+    const T* emat_sym;     // packed upper triangles [(e*2 + k)*P(P+1)/2 + t]
     int nelem;
     int dofs;
 };
@@ -39,16 +41,19 @@ enum Form { kFormHeat = 0, kFormProject = 1 };
 //   kFormHeat:    F = A(kappa) c, A = M - dt K(kappa)    (c: coefficients, FP64)
 //   kFormProject: F = int f0 v_a                         (f0 kind in f0)
 // F must be zero on entry (the kernel accumulates).  l2_part receives one
-// partial sum per workgroup (reduced by reduce_partials).
+// partial sum per workgroup (reduced by reduce_partials); their count goes
+// to *nparts, and a launch that would need more than part_cap is refused
+// (hipErrorInvalidValue) before anything runs.
 template <typename T>
 hipError_t launch_assemble(int p, int form, int f0, const double* c, T* F, const Tables<T>& tb, double dt,
-                           const double* kappa, double* l2_part, int* nparts, hipStream_t s);
+                           const double* kappa, double* l2_part, int part_cap, int* nparts, hipStream_t s);
 
-// dL/dc_n and dL/dkappa of one step given mu = M^-T lam:
-//   lam_out = A(kappa) mu,  dkappa[e] += -dt sum_q J W grad c_n . grad mu
-hipError_t launch_assemble_adjoint(int p, const double* c_n, const double* mu, double* lam_out,
-                                   const Tables<double>& tb, double dt, const double* kappa, double* dkappa,
-                                   hipStream_t s);
+// Adjoint assembly given mu = M^-T lam:  lam_out = A(kappa) mu (A symmetric)
+// and, when dkappa != nullptr, dkappa[e] += -dt c_n,e^T K_e mu_e in the same
+// pass.  lam_out must be zero on entry; l2_part as in launch_assemble.
+hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, double* lam_out, const Tables<double>& tb,
+                               double dt, const double* kappa, double* dkappa, double* l2_part, int part_cap,
+                               int* nparts, hipStream_t s);
 
 // Deterministic sum of n partials into out[0].
 hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStream_t s);
@@ -61,7 +66,9 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
 // r = A c - M x in FP64 with the 1-D banded matrices (constant kappa):
 // A = M - dt*kappa*K  where M = Mz(x)My(x)Mx, K = Mz(x)My(x)Kx + Mz(x)Ky(x)Mx + Kz(x)My(x)Mx.
 // x is read in precision TX (x == nullptr means x = 0); r is written in
-// precision TR.
+// precision TR.  With l2_part != nullptr each workgroup also writes its
+// partial sum of c^T M c; kron_residual_partials gives their count (for
+// launch_reduce_partials).
 struct BandSet {
     const double* mass[3];   // row tables: [i][t] = M(i, i-p+t), t = 0..2p, zero outside
     const double* stiff[3];  // [i][t] = K(i, i-p+t)
@@ -69,15 +76,10 @@ struct BandSet {
 };
 template <typename TX, typename TR>
 hipError_t launch_kron_residual(int p, const double* c, const TX* x, TR* r, const BandSet& bs, double dtk,
-                                double* work0, double* work1, double* work2, hipStream_t s);
+                                double* l2_part, hipStream_t s);
+int kron_residual_partials(const BandSet& bs);
 
-// out[0] = sum_i a_i b_i (deterministic: per-block partials, then one block)
-hipError_t launch_dot(const double* a, const double* b, int64_t n, double* part, int part_cap, double* out,
-                      hipStream_t s);
-
-// elementwise helpers
-hipError_t launch_convert_d2f(const double* in, float* out, int64_t n, hipStream_t s);
+// out = (a ? a : out) + (b ? b : 0), FP32 operands widened to FP64
 hipError_t launch_add_f2d(const float* a, const float* b, double* out, int64_t n, hipStream_t s);
-hipError_t launch_check_finite(const double* v, int* flag, hipStream_t s);
 
 }  // namespace ads

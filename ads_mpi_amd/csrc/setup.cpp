@@ -183,6 +183,29 @@ std::string build_direction(int n, int p, Direction& d)
         }
     }
 
+    // element matrices
+    const int nt = P * (P + 1) / 2;
+    d.emat.assign(static_cast<size_t>(d.nelem) * P * 2 * P, 0.0);
+    d.emat_sym.assign(static_cast<size_t>(d.nelem) * 2 * nt, 0.0);
+    for (int e = 0; e < d.nelem; ++e)
+        for (int a = 0; a < P; ++a)
+            for (int b = 0; b < P; ++b) {
+                double m = 0.0, k = 0.0;
+                for (int iq = 0; iq < q; ++iq) {
+                    const double* t = &d.tab[(static_cast<size_t>(e) * q + iq) * P * 2];
+                    const double jw = d.jac[e] * d.wts[iq];
+                    m += (t[a * 2] * t[b * 2]) * jw;
+                    k += (t[a * 2 + 1] * t[b * 2 + 1]) * jw;
+                }
+                d.emat[((static_cast<size_t>(e) * P + a) * 2 + 0) * P + b] = m;
+                d.emat[((static_cast<size_t>(e) * P + a) * 2 + 1) * P + b] = k;
+                if (a <= b) {
+                    const int t = a * P - a * (a - 1) / 2 + (b - a);
+                    d.emat_sym[(static_cast<size_t>(e) * 2 + 0) * nt + t] = m;
+                    d.emat_sym[(static_cast<size_t>(e) * 2 + 1) * nt + t] = k;
+                }
+            }
+
     // 1-D mass and stiffness matrices, band form (half-bandwidth p)
     const int bw = 2 * p + 1;
     d.mass_band.assign(static_cast<size_t>(d.dofs) * bw, 0.0);

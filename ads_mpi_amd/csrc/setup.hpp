@@ -24,6 +24,14 @@ struct Direction {
     std::vector<double> pts;     // X(iq, e) at [e*q + iq]
     std::vector<double> tab;     // NN(k, a, iq, e) at [((e*q + iq)*(p+1) + a)*2 + k]
 
+    // 1-D element matrices by Gauss quadrature over the element's basis
+    // values: Me(a, b) = sum_q N_a N_b J W_q, Ke(a, b) = sum_q N'_a N'_b J W_q
+    // (bitwise symmetric).  emat holds rows, [((e*P + a)*2 + k)*P + b] with
+    // k = 0 for Me and 1 for Ke; emat_sym the packed upper triangles,
+    // [(e*2 + k)*P(P+1)/2 + t], t = a*P - a(a-1)/2 + (b - a) for a <= b.
+    std::vector<double> emat;
+    std::vector<double> emat_sym;
+
     // 1-D matrices in band form, row-major by column: band[j*(2p+1) + (p + i - j)] = A(i, j)
     std::vector<double> mass_band;
     std::vector<double> stiff_band;

@@ -7,8 +7,8 @@
 //   k_sweep_rows       the same along x, where a line is a contiguous row: a
 //                      wave stages 64 rows through LDS tiles (the on-device
 //                      replacement of Gather/ReorderRHSFor*/Scatter).
-//   k_band_*           FP64 Kronecker residual r = A c - M x for the
-//                      FP32 + refinement form.
+//   k_kron_residual    FP64 Kronecker residual r = A c - M x for the
+//                      FP32 + refinement forms, one streaming pass.
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -345,147 +345,159 @@ __global__ __launch_bounds__(kWave) void k_sweep_rows(T* __restrict__ data, int 
 }
 
 // ---------------------------------------------------------------------------
-// Kronecker residual r = M(c - x) - dtk K c, staged per axis:
-//   X: e = Mx(c - x) - dtk Kx c,   m = Mx c       (thread per entry; the
-//                                                   x neighbours share lines)
-//   Y: f = My e - dtk Ky m,        g = My m       (lane per y line, streamed)
-//   Z: r = Mz f - dtk Kz g                        (lane per z line, streamed)
+// Kronecker residual r = A c - M x, A = M - dtk K (constant kappa), in FP64:
+//   X: e = Mx(c - x) - dtk Kx c,   m = Mx c
+//   Y: f = My e - dtk Ky m,        g = My m
+//   Z: r = Mz f - dtk Kz g,        and M c = Mz g for l2 = c^T M c
+// in one pass.  A workgroup of four waves owns 64 x columns and four y rows
+// (one wave each) and streams a z segment: per input plane it stages the
+// c (and c - x) tile with its halos (64+2p by 4+2p) in LDS, applies the X
+// stage to all tile rows, the Y stage per wave, and keeps the last 2p+1 Y
+// results of its lane in registers for the Z stage.  HBM traffic is about one
+// read of c and x and one write of r per DOF.
 // Row tables hold A(i, i-KL+t), t = 0..2KL, zero outside the matrix.
 // ---------------------------------------------------------------------------
-template <int P, typename TX>
-__global__ __launch_bounds__(256) void k_band_x(const double* __restrict__ c, const TX* __restrict__ x,
-                                                 double* __restrict__ e, double* __restrict__ m,
-                                                 const double* __restrict__ mrow, const double* __restrict__ krow,
-                                                 int nx, double dtk, long D)
+constexpr int kResRows = 4;
+
+struct ResDims {
+    int nx, ny, nz;
+    int xchunks, yblocks, zseg;
+};
+
+// consecutive workgroups go to the 8 XCDs round robin; give each XCD a
+// contiguous run of tiles so y-neighbours (which share halo rows) share an L2
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    constexpr int kXcds = 8;
+    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds, s = b / kXcds;
+    return k * q + min(k, r) + s;
+}
+
+template <int P, typename TX, typename TR>
+__global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
+    const double* __restrict__ c, const TX* __restrict__ x, TR* __restrict__ r, const double* __restrict__ mx_rows,
+    const double* __restrict__ kx_rows, const double* __restrict__ my_rows, const double* __restrict__ ky_rows,
+    const double* __restrict__ mz_rows, const double* __restrict__ kz_rows, double dtk, ResDims dm,
+    double* __restrict__ l2_part)
 {
     constexpr int KL = P - 1, BW = 2 * KL + 1;
-    const long idx = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-    if (idx >= D) return;
-    const int i = static_cast<int>(idx % nx);
-    double se = 0.0, sm = 0.0, sk = 0.0;
+    constexpr int TYH = kResRows + 2 * KL, TXH = kWave + 2 * KL, NTH = kWave * kResRows;
+    __shared__ double tc[TYH][TXH];  // c tile, rows y0-KL.., columns x0-KL..
+    __shared__ double td[TYH][TXH];  // c - x tile
+    __shared__ double te[TYH][kWave];
+    __shared__ double tm[TYH][kWave];
+    __shared__ double l2w[kResRows];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int xc = tile % dm.xchunks;
+    const int yb = (tile / dm.xchunks) % dm.yblocks;
+    const int seg = tile / (dm.xchunks * dm.yblocks);
+    const int x0 = xc * kWave, y0 = yb * kResRows;
+    const int gx = x0 + lane, gy = y0 + wv;
+    const int zs = seg * dm.zseg;
+    const int ze = min(zs + dm.zseg, dm.nz);
+    const bool xin = gx < dm.nx, yin = gy < dm.ny;
+    const long sxy = static_cast<long>(dm.nx) * dm.ny;
+
+    double mxr[BW], kxr[BW];
 #pragma unroll
     for (int t = 0; t < BW; ++t) {
-        const int j = i - KL + t;
-        if (j < 0 || j >= nx) continue;
-        const double cv = c[idx - KL + t];
-        const double dv = x ? cv - static_cast<double>(x[idx - KL + t]) : cv;
-        const double am = mrow[i * BW + t], ak = krow[i * BW + t];
-        se = fma(am, dv, se);
-        sm = fma(am, cv, sm);
-        sk = fma(ak, cv, sk);
+        mxr[t] = xin ? mx_rows[static_cast<size_t>(gx) * BW + t] : 0.0;
+        kxr[t] = xin ? kx_rows[static_cast<size_t>(gx) * BW + t] : 0.0;
     }
-    e[idx] = fma(-dtk, sk, se);
-    m[idx] = sm;
-}
+    const int gyc = min(gy, dm.ny - 1);
+    const double* __restrict__ myr = my_rows + static_cast<size_t>(gyc) * BW;
+    const double* __restrict__ kyr = ky_rows + static_cast<size_t>(gyc) * BW;
 
-// out0 = M a - dtk K b  (to out0, or rounded to r_out when r_out != nullptr)
-// out1 = M b            (when out1 != nullptr)
-template <int P, typename TR>
-__global__ __launch_bounds__(256) void k_band_lines(const double* __restrict__ a_in, const double* __restrict__ b_in,
-                                                     double* __restrict__ out0, double* __restrict__ out1,
-                                                     TR* __restrict__ r_out, const double* __restrict__ mrow_g,
-                                                     const double* __restrict__ krow_g, int n, int ni, long stride_j,
-                                                     long stride_k, int nlines, double dtk)
-{
-    constexpr int KL = P - 1, BW = 2 * KL + 1, CH = 16;
-    extern __shared__ unsigned char smem[];
-    double* mrow = reinterpret_cast<double*>(smem);
-    double* krow = mrow + n * BW;
-    for (int q = threadIdx.x; q < n * BW; q += blockDim.x) {
-        mrow[q] = mrow_g[q];
-        krow[q] = krow_g[q];
-    }
-    __syncthreads();
-    const int L = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= nlines) return;
-    const long base = (L % ni) + static_cast<long>(L / ni) * stride_j;
-    double wa[BW], wb[BW];
+    double fw[BW], gw[BW], cw[KL + 1];  // z windows: f, g and the centre c
 #pragma unroll
-    for (int t = 0; t < BW; ++t) {
-        const int k = t - KL;
-        wa[t] = (k >= 0 && k < n) ? a_in[base + k * stride_k] : 0.0;
-        wb[t] = (k >= 0 && k < n) ? b_in[base + k * stride_k] : 0.0;
-    }
-    const int nch = (n + CH - 1) / CH;
-    auto fetch = [&](int ch, double* ba, double* bb) {
+    for (int t = 0; t < BW; ++t) fw[t] = gw[t] = 0.0;
 #pragma unroll
-        for (int q = 0; q < CH; ++q) {
-            const int k = ch * CH + q + KL + 1;  // input entering the window after output ch*CH+q
-            const bool ok = ch < nch && k < n;
-            ba[q] = ok ? a_in[base + k * stride_k] : 0.0;
-            bb[q] = ok ? b_in[base + k * stride_k] : 0.0;
-        }
-    };
-    double ca[CH], cb[CH], na[CH], nb[CH];
-    fetch(0, ca, cb);
-    for (int ch = 0; ch < nch; ++ch) {
-        fetch(ch + 1, na, nb);
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-            const int j = ch * CH + q;
-            if (j < n) {
-                double s0 = 0.0, s1 = 0.0, sk = 0.0;
-#pragma unroll
-                for (int t = 0; t < BW; ++t) {
-                    const double am = mrow[j * BW + t], ak = krow[j * BW + t];
-                    s0 = fma(am, wa[t], s0);
-                    s1 = fma(am, wb[t], s1);
-                    sk = fma(ak, wb[t], sk);
-                }
-                const double o0 = fma(-dtk, sk, s0);
-                if (r_out) r_out[base + j * stride_k] = static_cast<TR>(o0);
-                else out0[base + j * stride_k] = o0;
-                if (out1) out1[base + j * stride_k] = s1;
-            }
-#pragma unroll
-            for (int t = 0; t < BW - 1; ++t) {
-                wa[t] = wa[t + 1];
-                wb[t] = wb[t + 1];
-            }
-            wa[BW - 1] = ca[q];
-            wb[BW - 1] = cb[q];
-        }
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-            ca[q] = na[q];
-            cb[q] = nb[q];
-        }
-    }
-}
+    for (int t = 0; t <= KL; ++t) cw[t] = 0.0;
+    double l2 = 0.0;
 
-__global__ __launch_bounds__(256) void k_dot_partials(const double* __restrict__ a, const double* __restrict__ b,
-                                                      long n, double* __restrict__ part)
-{
-    __shared__ double red[256];
-    double acc = 0.0;
-    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) acc = fma(a[i], b[i], acc);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    for (int zi = zs - KL; zi < ze + KL; ++zi) {
+        const bool zok = zi >= 0 && zi < dm.nz;
+        for (int i = tid; i < TYH * TXH; i += NTH) {
+            const int rr = i / TXH, cc = i - rr * TXH;
+            const int yy = y0 - KL + rr, xx = x0 - KL + cc;
+            const bool ok = zok && yy >= 0 && yy < dm.ny && xx >= 0 && xx < dm.nx;
+            const long at = xx + static_cast<long>(dm.nx) * yy + sxy * zi;
+            const double cv = ok ? c[at] : 0.0;
+            double dv = cv;
+            if (x) dv = ok ? cv - static_cast<double>(x[at]) : 0.0;
+            tc[rr][cc] = cv;
+            td[rr][cc] = dv;
+        }
         __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int n, double* __restrict__ out)
-{
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        for (int rr = wv; rr < TYH; rr += kResRows) {
+            double se = 0.0, sm = 0.0, sk = 0.0;
+#pragma unroll
+            for (int t = 0; t < BW; ++t) {
+                const double cv = tc[rr][lane + t], dv = td[rr][lane + t];
+                se = fma(mxr[t], dv, se);
+                sm = fma(mxr[t], cv, sm);
+                sk = fma(kxr[t], cv, sk);
+            }
+            te[rr][lane] = fma(-dtk, sk, se);
+            tm[rr][lane] = sm;
+        }
+        const double c_here = tc[wv + KL][lane + KL];
         __syncthreads();
+        double f = 0.0, fk = 0.0, g = 0.0;
+#pragma unroll
+        for (int t = 0; t < BW; ++t) {
+            const double e = te[wv + t][lane], m = tm[wv + t][lane];
+            f = fma(myr[t], e, f);
+            fk = fma(kyr[t], m, fk);
+            g = fma(myr[t], m, g);
+        }
+        f = fma(-dtk, fk, f);
+#pragma unroll
+        for (int t = 0; t < BW - 1; ++t) {
+            fw[t] = fw[t + 1];
+            gw[t] = gw[t + 1];
+        }
+        fw[BW - 1] = f;
+        gw[BW - 1] = g;
+#pragma unroll
+        for (int t = 0; t < KL; ++t) cw[t] = cw[t + 1];
+        cw[KL] = c_here;
+        const int zo = zi - KL;  // the window now covers planes zo-KL..zo+KL
+        if (zo >= zs) {
+            const double* __restrict__ mzr = mz_rows + static_cast<size_t>(zo) * BW;
+            const double* __restrict__ kzr = kz_rows + static_cast<size_t>(zo) * BW;
+            double o = 0.0, ok = 0.0;
+#pragma unroll
+            for (int t = 0; t < BW; ++t) {
+                o = fma(mzr[t], fw[t], o);
+                ok = fma(kzr[t], gw[t], ok);
+            }
+            o = fma(-dtk, ok, o);
+            if (xin && yin) r[gx + static_cast<long>(dm.nx) * gy + sxy * zo] = static_cast<TR>(o);
+            if (l2_part) {
+                double mc = 0.0;
+#pragma unroll
+                for (int t = 0; t < BW; ++t) mc = fma(mzr[t], gw[t], mc);
+                l2 = fma(cw[0], mc, l2);
+            }
+        }
     }
-    if (threadIdx.x == 0) out[0] = red[0];
-}
-
-__global__ __launch_bounds__(256) void k_convert_d2f(const double* __restrict__ in, float* __restrict__ out, long n)
-{
-    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-    if (i < n) out[i] = static_cast<float>(in[i]);
+    if (l2_part) {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) l2 += __shfl_xor(l2, off, kWave);
+        if (lane == 0) l2w[wv] = l2;
+        __syncthreads();
+        if (tid == 0) {
+            double sum = 0.0;
+#pragma unroll
+            for (int w = 0; w < kResRows; ++w) sum += l2w[w];
+            l2_part[blockIdx.x] = sum;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_add_f2d(const float* __restrict__ a, const float* __restrict__ b,
@@ -494,7 +506,7 @@ __global__ __launch_bounds__(256) void k_add_f2d(const float* __restrict__ a, co
     const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     const double base = a ? static_cast<double>(a[i]) : out[i];
-    out[i] = base + static_cast<double>(b[i]);
+    out[i] = b ? base + static_cast<double>(b[i]) : base;
 }
 
 template <typename T, int KL>
@@ -521,28 +533,32 @@ hipError_t kron_solve_p(T* F, const SweepTables<T> sw[3], bool trans, hipStream_
     return hipGetLastError();
 }
 
-template <int P, typename TX, typename TR>
-hipError_t residual_p(const double* c, const TX* x, TR* r, const BandSet& bs, double dtk, double* w0, double* w1,
-                      double* w2, double* w3, hipStream_t s)
+int residual_blocks(const BandSet& bs, ResDims* dm_out)
 {
-    const int nx = bs.n[0], ny = bs.n[1], nz = bs.n[2];
-    const long D = static_cast<long>(nx) * ny * nz;
-    const long sxy = static_cast<long>(nx) * ny;
-    hipLaunchKernelGGL((k_band_x<P, TX>), dim3(static_cast<unsigned>((D + 255) / 256)), dim3(256), 0, s, c, x, w0, w1,
-                       bs.mass[0], bs.stiff[0], nx, dtk, D);
-    // Y: f = My e - dtk Ky m -> w2 ; g = My m -> w3   (lines (x, z))
-    const int ylines = nx * nz;
-    const size_t by = 2 * sizeof(double) * (2 * P - 1) * static_cast<size_t>(ny);
-    const size_t bz = 2 * sizeof(double) * (2 * P - 1) * static_cast<size_t>(nz);
-    if (by > 61440 || bz > 61440) return hipErrorInvalidValue;  // line too long for the LDS tables
-    hipLaunchKernelGGL((k_band_lines<P, TR>), dim3((ylines + 255) / 256), dim3(256), by, s, w0, w1, w2, w3,
-                       static_cast<TR*>(nullptr), bs.mass[1], bs.stiff[1], ny, nx, sxy, static_cast<long>(nx), ylines,
-                       dtk);
-    // Z: r = Mz f - dtk Kz g   (lines (x, y))
-    const int zlines = nx * ny;
-    hipLaunchKernelGGL((k_band_lines<P, TR>), dim3((zlines + 255) / 256), dim3(256), bz, s, w2, w3,
-                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), r, bs.mass[2], bs.stiff[2], nz,
-                       zlines, 0L, sxy, zlines, dtk);
+    ResDims dm{};
+    dm.nx = bs.n[0];
+    dm.ny = bs.n[1];
+    dm.nz = bs.n[2];
+    dm.xchunks = (dm.nx + kWave - 1) / kWave;
+    dm.yblocks = (dm.ny + kResRows - 1) / kResRows;
+    const long tiles = static_cast<long>(dm.xchunks) * dm.yblocks;
+    long nseg = (1024 + tiles - 1) / tiles;
+    if (nseg > dm.nz) nseg = dm.nz;
+    if (nseg < 1) nseg = 1;
+    dm.zseg = static_cast<int>((dm.nz + nseg - 1) / nseg);
+    nseg = (dm.nz + dm.zseg - 1) / dm.zseg;
+    if (dm_out) *dm_out = dm;
+    return static_cast<int>(tiles * nseg);
+}
+
+template <int P, typename TX, typename TR>
+hipError_t residual_p(const double* c, const TX* x, TR* r, const BandSet& bs, double dtk, double* l2_part,
+                      hipStream_t s)
+{
+    ResDims dm;
+    const int blocks = residual_blocks(bs, &dm);
+    hipLaunchKernelGGL((k_kron_residual<P, TX, TR>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, r,
+                       bs.mass[0], bs.stiff[0], bs.mass[1], bs.stiff[1], bs.mass[2], bs.stiff[2], dtk, dm, l2_part);
     return hipGetLastError();
 }
 
@@ -563,51 +579,29 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
 template hipError_t launch_kron_solve<double>(int, double*, const SweepTables<double>[3], bool, hipStream_t);
 template hipError_t launch_kron_solve<float>(int, float*, const SweepTables<float>[3], bool, hipStream_t);
 
+int kron_residual_partials(const BandSet& bs) { return residual_blocks(bs, nullptr); }
+
 template <typename TX, typename TR>
 hipError_t launch_kron_residual(int p, const double* c, const TX* x, TR* r, const BandSet& bs, double dtk,
-                                double* w0, double* w1, double* w2, hipStream_t s)
+                                double* l2_part, hipStream_t s)
 {
-    // the Y stage writes (f, g); r's own storage is not used as a temporary,
-    // so a fourth buffer is carved from w2's tail by the caller: w2 holds 2*D.
-    const long D = static_cast<long>(bs.n[0]) * bs.n[1] * bs.n[2];
-    double* w3 = w2 + D;
     switch (p) {
-    case 1: return residual_p<2, TX, TR>(c, x, r, bs, dtk, w0, w1, w2, w3, s);
-    case 2: return residual_p<3, TX, TR>(c, x, r, bs, dtk, w0, w1, w2, w3, s);
-    case 3: return residual_p<4, TX, TR>(c, x, r, bs, dtk, w0, w1, w2, w3, s);
-    case 4: return residual_p<5, TX, TR>(c, x, r, bs, dtk, w0, w1, w2, w3, s);
+    case 1: return residual_p<2, TX, TR>(c, x, r, bs, dtk, l2_part, s);
+    case 2: return residual_p<3, TX, TR>(c, x, r, bs, dtk, l2_part, s);
+    case 3: return residual_p<4, TX, TR>(c, x, r, bs, dtk, l2_part, s);
+    case 4: return residual_p<5, TX, TR>(c, x, r, bs, dtk, l2_part, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 template hipError_t launch_kron_residual<float, float>(int, const double*, const float*, float*, const BandSet&,
-                                                       double, double*, double*, double*, hipStream_t);
+                                                       double, double*, hipStream_t);
 template hipError_t launch_kron_residual<double, float>(int, const double*, const double*, float*, const BandSet&,
-                                                        double, double*, double*, double*, hipStream_t);
+                                                        double, double*, hipStream_t);
 template hipError_t launch_kron_residual<double, double>(int, const double*, const double*, double*,
-                                                         const BandSet&, double, double*, double*, double*,
-                                                         hipStream_t);
+                                                         const BandSet&, double, double*, hipStream_t);
 template hipError_t launch_kron_residual<float, double>(int, const double*, const float*, double*, const BandSet&,
-                                                        double, double*, double*, double*, hipStream_t);
-
-hipError_t launch_dot(const double* a, const double* b, int64_t n, double* part, int part_cap, double* out,
-                      hipStream_t s)
-{
-    int blocks = static_cast<int>((n + 255) / 256);
-    if (blocks > 2048) blocks = 2048;
-    if (blocks > part_cap) blocks = part_cap;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_dot_partials, dim3(blocks), dim3(256), 0, s, a, b, static_cast<long>(n), part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, blocks, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_convert_d2f(const double* in, float* out, int64_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_convert_d2f, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, in, out,
-                       static_cast<long>(n));
-    return hipGetLastError();
-}
+                                                        double, double*, hipStream_t);
 
 hipError_t launch_add_f2d(const float* a, const float* b, double* out, int64_t n, hipStream_t s)
 {

@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
-"""Headline benchmark: one explicit ADS heat step (quadrature RHS assembly +
-x/y/z banded mass-matrix sweeps) at 256^3 elements, p = 3 (259^3 DOFs).
+"""Headline benchmark: one explicit ADS heat step (RHS assembly by element
+quadrature + x/y/z banded mass-matrix sweeps) at 256^3 elements, p = 3
+(259^3 DOFs).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
 Prints ONE JSON line.  ``value`` is FP64 DOF-updates/s of the plain step with
 the state resident in HBM (timed with HIP events on the stream the library
 runs on, bracketed by device synchronisation).  Alongside it:
-  * ``fp32_ir``: the precision form as literally specified (FP32 quadrature
+  * ``fp32_ir``: the precision form as literally specified (FP32 element
     assembly + FP32 sweeps, then one FP64 Kronecker-residual refinement) on the
     same state, its speed-up and its error against the FP64 step;
-  * ``fp32_ir_kron``: the precision form without quadrature (two passes of
-    FP64 Kronecker residual + FP32 sweeps from a zero start; constant kappa);
+  * ``fp32_ir_kron``: the precision form without element assembly (two passes
+    of FP64 Kronecker residual + FP32 sweeps from a zero start; constant kappa);
   * ``grad_fwd``: wall ratio of a checkpointed gradient (recompute + adjoint)
     to the forward rollout at 96^3 elements, p = 2, 50 steps, checkpoint 10;
-  * ``roofline``: the assembly kernel's FP64 throughput against MI355X peak;
+  * ``roofline``: the heat assembly kernel (the step's dominant kernel) timed
+    alone with HIP events on the library stream (ads_time_kernel), its
+    algorithmic FP64 flops against MI355X's FP64 peak; ``kron_residual`` the
+    same for the fused residual kernel of the fast precision form, by bytes;
   * ``cpu_baseline``: the oracle's reference-structured CPU path (per-point
     integrand loops, rebuilt 1-D factorisations, per-line solves) on a
     bounded sample of the same step, extrapolated to DOF/s.
@@ -39,17 +43,18 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 
-def assembly_flops(E, P):
-    """Algorithmic flops of one assembly launch: per element 18 P^4 FMAs of
-    sum factorisation (9 P^4 forward for u and grad u, 9 P^4 transposed) and
-    ~12 flops per quadrature point for the integrand: 36 P^4 + 12 P^3."""
-    return E * (36 * P ** 4 + 12 * P ** 3)
+def heat_kernel_flops(E, P):
+    """Algorithmic flops of one heat-assembly launch: per element 8 P^4 FMAs
+    (x stage 2 P^4, z stage 3 P^4, y stage 3 P^4 for M c and K c of the
+    element operator) and 2 P^3 FMAs (combining M c - dt kappa K c, and
+    c . M c for l2): 16 P^4 + 4 P^3 flops."""
+    return E * (16 * P ** 4 + 4 * P ** 3)
 
 
-def step_bytes(D):
-    """Algorithmic HBM bytes of one step: read c and write F (16 B/DOF), then
-    three sweeps each reading and writing the lines once (48 B/DOF)."""
-    return 64 * D
+def residual_bytes(D):
+    """Algorithmic HBM bytes of the first residual pass: read c (FP64) and
+    write r (FP32) once: 12 B/DOF."""
+    return 12 * D
 
 
 def cpu_baseline(n, p, dt, budget_s=8.0):
@@ -157,9 +162,9 @@ def main():
     forms = {}
     for key, prec, refine, what in (
             ("fp32_ir", _lib.ADS_PREC_FP32_IR, 1,
-             "FP32 quadrature assembly + FP32 sweeps, then 1 FP64 Kronecker-residual refinement"),
+             "FP32 element assembly + FP32 sweeps, then 1 FP64 Kronecker-residual refinement"),
             ("fp32_ir_kron", _lib.ADS_PREC_FP32_IR_KRON, 2,
-             "zero start, 2 passes of FP64 Kronecker residual + FP32 sweeps (no quadrature)")):
+             "zero start, 2 passes of fused FP64 Kronecker residual + FP32 sweeps (no element assembly)")):
         _lib.check(L.ads_step_d(h, vp(cdev), vp(got), None, dt, prec, refine, None), h)
         torch.cuda.synchronize()
         rel_l2 = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
@@ -174,29 +179,27 @@ def main():
         forms[key] = {"value": D / (ms * 1e-3), "ms_per_step": ms, "refine": refine, "form": what,
                       "speedup_vs_fp64": ms64 / ms, "rel_l2_vs_fp64": rel_l2, "max_abs_err_vs_fp64": max_err}
 
-    # ---- assembly kernel alone (roofline) ------------------------------
-    F = torch.empty(D, dtype=torch.float64, device="cuda")
-    l2d = torch.empty(1, dtype=torch.float64, device="cuda")
-    for _ in range(2):
-        _lib.check(L.ads_form_rhs_d(h, vp(cdev), None, dt, vp(F), vp(l2d)), h)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        _lib.check(L.ads_form_rhs_d(h, vp(cdev), None, dt, vp(F), vp(l2d)), h)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_asm = e0.elapsed_time(e1) / args.steps
+    # ---- kernels alone (roofline): HIP events around the launches only ----
+    ms_k = ctypes.c_double()
+    _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_HEAT_FP64, dt, 2, ctypes.byref(ms_k)), h)
+    _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_HEAT_FP64, dt, args.steps, ctypes.byref(ms_k)), h)
+    ms_asm = ms_k.value
+    _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_KRON_RESIDUAL, dt, 2, ctypes.byref(ms_k)), h)
+    _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_KRON_RESIDUAL, dt, args.steps, ctypes.byref(ms_k)), h)
+    ms_res = ms_k.value
     if args.profile:
         # calibration dispatch for the PMC byte counters: D doubles read, D written
+        F = torch.empty_like(cdev)
         F.copy_(cdev)
         torch.cuda.synchronize()
-    flops = assembly_flops(E, P)
+    flops = heat_kernel_flops(E, P)
     achieved = flops / (ms_asm * 1e-3) / 1e12
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "assemble_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "heat_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    res_gbs = residual_bytes(D) / (ms_res * 1e-3) / 1e9
 
     adss.Cleanup(ads, data)
 
@@ -240,7 +243,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "heat step, 256x256x256 elements, p=3, 259^3 DOFs, dt=1e-7, FP64 "
-                               "(quadrature assembly + x/y/z banded sweeps)",
+                               "(element-quadrature RHS assembly + x/y/z banded sweeps)",
                    "elements": [256, 256, 256], "p": 3, "dofs": D, "dt": dt},
         "wall_s_timed": wall,
         "l2norm": l2.value,
@@ -249,9 +252,12 @@ def main():
         "grad_fwd": grad,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": "k_assemble<double,P=4,heat> (+ its zero-fill and l2 reduce)",
+                     "kernel": "k_heat<double,P=4> (FP64 VALU; MI355X's FP64 vector and matrix peaks are equal)",
                      "ms_per_launch": ms_asm, "flops_per_launch": flops,
-                     "step_hbm_frac": step_bytes(D) / (ms64 * 1e-3) / 1e9 / PEAK_HBM_GBS},
+                     "flops_per_element": 16 * P ** 4 + 4 * P ** 3, "elements_per_launch": E},
+        "kron_residual": {"bound": "hbm", "achieved": res_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": res_gbs / PEAK_HBM_GBS, "kernel": "k_kron_residual<P=4,x=0,r=FP32>",
+                          "ms_per_launch": ms_res, "bytes_per_launch": residual_bytes(D)},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(n, p, dt),
     }
     print(json.dumps(line), flush=True)

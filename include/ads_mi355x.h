@@ -39,8 +39,12 @@ extern "C" {
 
 /* precision forms */
 #define ADS_PREC_FP64 0    /* FP64 assembly and sweeps                            */
-#define ADS_PREC_FP32_IR 1 /* FP32 quadrature assembly and sweeps + FP64 Kronecker residual refinement */
+#define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 Kronecker residual refinement (constant kappa) */
 #define ADS_PREC_FP32_IR_KRON 2 /* FP32 sweeps on FP64 Kronecker residuals from a zero start (constant kappa) */
+
+/* kernels for ads_time_kernel */
+#define ADS_KERNEL_HEAT_FP64 0      /* heat RHS assembly, FP64 */
+#define ADS_KERNEL_KRON_RESIDUAL 1  /* fused FP64 Kronecker residual, first pass (x = 0, FP32 out, l2) */
 
 typedef struct ads_ctx ads_ctx;
 
@@ -134,7 +138,7 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
  * gradients).  Given c_n and lam_next = dL/dc_{n+1}:
  *   mu = M^-T lam_next,  lam_out = A(kappa) mu = dL/dc_n,
  *   dkappa[e] += -dt sum_{q in e} J W grad c_n . grad mu   (if dkappa_d != NULL).
- * lam_out may equal lam_next. */
+ * lam_out may equal lam_next, not c_n. */
 int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next_d,
                        const double* kappa_d, double dt, double* lam_out_d, double* dkappa_d);
 
@@ -154,6 +158,15 @@ int ads_fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out
 /* lognormal(mu, sigma) by Box-Muller on the same generator (key offset
  * 0x8CB92BA72F3D8DD7, u1 = u_{2i}, u2 = u_{2i+1}). */
 int ads_fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double* out);
+
+/* ---- measurement ------------------------------------------------------------ */
+
+/* No reference counterpart (measurement support).  Times `launches`
+ * back-to-back launches of one kernel of the step (ADS_KERNEL_*) on the
+ * context's current coefficients with HIP events recorded on the context
+ * stream around the launches only (no fills, reductions or copies inside the
+ * events); results go to internal scratch.  ms_per_launch = elapsed / launches. */
+int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms_per_launch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,113 @@
+"""Differentiation form on the GPU: the checkpointed torch.autograd rollout
+(ads_mpi_amd/autograd.py over ads_step_d / ads_step_adjoint_d).
+
+  * against the oracle's FP64 adjoint (reference gradient) at a size the CPU
+    finishes in seconds, <= 1e-11 relative;
+  * against central finite differences (eps = 1e-6) on 32 random directions
+    at the config 4 size (96^3 elements, p = 2, 50 steps, checkpoint 10,
+    lognormal(0, 0.5) diffusivity from seed 3, loss 0.5 ||c_T - c_obs||^2 with
+    c_obs a forward run under a seed-4 diffusivity).  The directions are
+    standard normal in every component (not normalised): a unit vector over
+    ~10^6 unknowns would move each one by ~1e-9, about where the rounding of
+    an FP64 rollout (~1e-12 per component here) already limits the
+    difference quotient to ~1e-3.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from ads_mpi_amd import adss  # noqa: E402
+from ads_mpi_amd.autograd import HeatSolver, heat_rollout  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def test_gradient_vs_oracle_adjoint(orc):
+    n, p, dt, steps, ck = 25, 2, 1e-5, 6, 4
+    say("case adjoint vs oracle", n, p, steps)
+    S = orc.Setup(n, p)
+    solver = HeatSolver(n, p)
+    try:
+        c0n = adss.uniform(11, S.D, -1.0, 1.0)
+        kn = adss.lognormal(3, S.E, 0.0, 0.5)
+        obs = adss.uniform(12, S.D, -1.0, 1.0)
+        c0 = torch.from_numpy(c0n).cuda().requires_grad_(True)
+        kap = torch.from_numpy(kn).cuda().requires_grad_(True)
+        cT = heat_rollout(c0, kap, solver, dt, steps, ck)
+        loss = 0.5 * ((cT - torch.from_numpy(obs).cuda()) ** 2).sum()
+        loss.backward()
+        # oracle: forward trajectory then the adjoint steps backward
+        traj = [c0n]
+        for _ in range(steps):
+            traj.append(S.step(traj[-1], dt, kappa=kn)[0])
+        lam = traj[-1] - obs
+        dk = np.zeros(S.E)
+        for j in range(steps - 1, -1, -1):
+            lam = S.step_adjoint(traj[j], lam, dt, kappa=kn, dkappa=dk)
+        assert abs(float(loss) - 0.5 * np.sum((traj[-1] - obs) ** 2)) <= 1e-12 * float(loss)
+        gc = c0.grad.cpu().numpy()
+        gk = kap.grad.cpu().numpy()
+        assert np.linalg.norm(gc - lam) <= 1e-11 * np.linalg.norm(lam)
+        assert np.linalg.norm(gk - dk) <= 1e-11 * np.linalg.norm(dk)
+    finally:
+        solver.close()
+
+
+def test_gradient_vs_finite_differences_config4():
+    n, p, dt, steps, ck = 97, 2, 1e-7, 50, 10
+    say("case FD config4")
+    solver = HeatSolver(n, p)
+    try:
+        D, E = solver.D, solver.E
+        c0n = adss.uniform(3, D, -1.0, 1.0)
+        k_true = torch.from_numpy(adss.lognormal(4, E, 0.0, 0.5)).cuda()
+        kap0 = torch.from_numpy(adss.lognormal(3, E, 0.0, 0.5)).cuda()
+        c0 = torch.from_numpy(c0n).cuda()
+        with torch.no_grad():
+            c_obs = heat_rollout(c0, k_true, solver, dt, steps, ck)
+
+        def final(c, k):
+            with torch.no_grad():
+                return heat_rollout(c, k, solver, dt, steps, ck)
+
+        cg = c0.clone().requires_grad_(True)
+        kg = kap0.clone().requires_grad_(True)
+        cT = heat_rollout(cg, kg, solver, dt, steps, ck)
+        loss = 0.5 * ((cT - c_obs) ** 2).sum()
+        loss.backward()
+        g_c, g_k = cg.grad, kg.grad
+        # rounding floor of the forward rollout: the same input twice (the
+        # assembly's atomic adds make the summation order vary)
+        noise = float(torch.linalg.norm(final(c0, kap0) - final(c0, kap0)))
+        gen = torch.Generator(device="cpu").manual_seed(7)
+        eps = 1e-6
+        worst = 0.0
+        for i in range(32):
+            vc = torch.randn(D, generator=gen, dtype=torch.float64).cuda()
+            vk = torch.randn(E, generator=gen, dtype=torch.float64).cuda()
+            # central difference of L = 0.5 ||c_T - c_obs||^2, formed as
+            # L+ - L- = 0.5 (r+ - r-) . (r+ + r-) so that the rounding of the
+            # two sums of squares does not swamp the O(eps) difference
+            rp = final(c0 + eps * vc, kap0 + eps * vk) - c_obs
+            rm = final(c0 - eps * vc, kap0 - eps * vk) - c_obs
+            fd = float(0.5 * ((rp - rm) * (rp + rm)).sum()) / (2 * eps)
+            ad = float((g_c * vc).sum() + (g_k * vk).sum())
+            err = abs(fd - ad) / max(abs(ad), 1e-30)
+            worst = max(worst, err)
+        say("FD worst relative error", worst, "rollout rounding floor (L2 of a repeat difference)", noise)
+        assert worst <= 1e-5
+    finally:
+        solver.close()

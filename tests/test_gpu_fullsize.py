@@ -1,0 +1,120 @@
+"""Full-size GPU checks through size-independent properties (the CPU oracle
+cannot run these sizes step by step in a test):
+
+  * Kronecker identity of the assembled right-hand side: with Q = p+1 Gauss
+    points the heat RHS equals (Mz(x)My(x)Mx - dt(Mz(x)My(x)Kx + Mz(x)Ky(x)Mx
+    + Kz(x)My(x)Mx)) c exactly, applied here on the CPU with the oracle's 1-D
+    banded matrices;
+  * the solve: M x = F to 1e-12 (CPU Kronecker apply of M);
+  * the projection of prod sin(pi x_i) has l2norm 1/8 to quadrature accuracy;
+  * the FP32 + refinement forms agree with FP64 to <= 1e-10.
+Sizes: 256^3 elements p = 3 (the headline workload) and 384x384x256 p = 3.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from ads_mpi_amd import _lib, adss  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def mats(orc, n, p):
+    U = orc.knots(n, p)
+    ne = n - p + 1
+    M = orc.band_to_dense(orc.mkbbt_band(U, p, n, ne, (1, 0, 0, 0)), p, p)
+    K = orc.band_to_dense(orc.mkbbt_band(U, p, n, ne, (0, 1, 0, 0)), p, p)
+    return sp.csr_matrix(M), sp.csr_matrix(K)
+
+
+def apply_axis(A, X, axis):
+    """Apply the 1-D matrix A along axis (0 = x, 1 = y, 2 = z) of X[z][y][x]."""
+    ax = 2 - axis
+    Y = np.moveaxis(X, ax, -1)
+    shp = Y.shape
+    out = (A @ Y.reshape(-1, shp[-1]).T).T
+    return np.moveaxis(out.reshape(shp), -1, ax)
+
+
+def vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@pytest.mark.parametrize("n,p", [((258, 258, 258), 3), ((386, 386, 258), 3)])
+def test_rhs_kronecker_identity_and_solve(orc, n, p):
+    say("case fullsize", n, p)
+    ads, data = adss.initialize(n, p)
+    L = _lib.load()
+    h = data.handle
+    try:
+        nx, ny, nz = data.dofs
+        D = data.D
+        c = adss.uniform(21, D, -1.0, 1.0)
+        dt = 1e-7
+        cd = torch.from_numpy(c).cuda()
+        F = torch.empty_like(cd)
+        torch.cuda.synchronize()
+        _lib.check(L.ads_form_rhs_d(h, vp(cd), None, dt, vp(F), None), h)
+        _lib.check(L.ads_synchronize(h), h)
+        Fg = F.cpu().numpy()
+        # x = M^-1 F on the GPU
+        _lib.check(L.ads_solve_d(h, vp(F), 0), h)
+        _lib.check(L.ads_synchronize(h), h)
+        xg = F.cpu().numpy()
+        say("gpu done; CPU Kronecker apply")
+        (Mx, Kx), (My, Ky), (Mz, Kz) = [mats(orc, n[d], p) for d in range(3)]
+        C = c.reshape(nz, ny, nx)
+        mx = apply_axis(Mx, C, 0)
+        kx = apply_axis(Kx, C, 0)
+        mym = apply_axis(My, mx, 1)
+        Fr = apply_axis(Mz, mym - dt * (apply_axis(My, kx, 1) + apply_axis(Ky, mx, 1)), 2) - dt * apply_axis(Kz, mym, 2)
+        err_f = np.linalg.norm(Fg - Fr.ravel()) / np.linalg.norm(Fr)
+        X = xg.reshape(nz, ny, nx)
+        Mxv = apply_axis(Mz, apply_axis(My, apply_axis(Mx, X, 0), 1), 2)
+        err_s = np.linalg.norm(Mxv.ravel() - Fg) / np.linalg.norm(Fg)
+        say("rhs rel err", err_f, "solve residual", err_s)
+        assert err_f <= 1e-12
+        assert err_s <= 1e-12
+    finally:
+        adss.Cleanup(ads, data)
+
+
+def test_projection_l2norm_and_precision_forms_256():
+    n, p, dt = 258, 3, 1e-7
+    say("case 256^3 projection + precision forms")
+    ads, data = adss.initialize(n, p)
+    L = _lib.load()
+    h = data.handle
+    try:
+        l2 = adss.Step(0, adss.HeatRHS(Dt=dt, f0=adss.ADS_F0_SINPROD), ads, data)
+        assert abs(l2 - 0.125) <= 1e-12
+        c = data.F + adss.uniform(2, data.D, -0.01, 0.01)
+        cd = torch.from_numpy(c).cuda()
+        ref = torch.empty_like(cd)
+        got = torch.empty_like(cd)
+        torch.cuda.synchronize()
+        _lib.check(L.ads_step_d(h, vp(cd), vp(ref), None, dt, _lib.ADS_PREC_FP64, 0, None), h)
+        for prec, refine in ((_lib.ADS_PREC_FP32_IR, 1), (_lib.ADS_PREC_FP32_IR_KRON, 2)):
+            _lib.check(L.ads_step_d(h, vp(cd), vp(got), None, dt, prec, refine, None), h)
+            _lib.check(L.ads_synchronize(h), h)
+            err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+            mx = float(torch.max(torch.abs(got - ref)))
+            say("prec", prec, "refine", refine, "rel L2", err, "max abs", mx)
+            assert err <= 1e-10
+    finally:
+        adss.Cleanup(ads, data)
