@@ -38,7 +38,8 @@ from ._lib import (ADS_F0_BUMP3D, ADS_F0_CONST, ADS_F0_SINPROD, ADS_PREC_FP32_IR
 __all__ = [
     "ADS_setup", "ADS_compute_data", "initialize", "Step", "MultiStep", "Cleanup", "Form3DRHS", "FormUn",
     "ComputeMatrix", "HeatRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
-    "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR",
+    "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR", "PlotParams", "EvalGrid", "VtkOutput",
+    "PrintSolution",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -274,6 +275,57 @@ def ComputeMatrix(ads, ads_data, direction, mix):
     m = np.asarray(mix, dtype=np.float64)
     check(L.ads_get_matrix(ads_data.handle, int(direction), _d(m), _d(out)), ads_data.handle)
     return out
+
+
+@dataclass
+class PlotParams:
+    """type(PlotParams) (src/plot.F90:14-22): sample ranges and counts."""
+
+    startx: float = 0.0
+    endx: float = 1.0
+    starty: float = 0.0
+    endy: float = 1.0
+    startz: float = 0.0
+    endz: float = 1.0
+    resx: int = 31
+    resy: int = 31
+    resz: int = 31
+
+
+def EvalGrid(ads_data, params):
+    """The spline of the current coefficients on the PlotParams grid
+    (SaveSplinePlot + EvalSpline, src/plot.F90:97-145, src/basis.F90:323-369),
+    evaluated on the GPU (ads_eval_grid).  Returns vals[iz, iy, ix]."""
+    L = _lib.load()
+    res = (ctypes.c_int * 3)(params.resx, params.resy, params.resz)
+    lo = (ctypes.c_double * 3)(params.startx, params.starty, params.startz)
+    hi = (ctypes.c_double * 3)(params.endx, params.endy, params.endz)
+    vals = np.zeros((params.resz, params.resy, params.resx))
+    check(L.ads_eval_grid(ads_data.handle, res, lo, hi, _d(vals)), ads_data.handle)
+    return vals
+
+
+def VtkOutput(filename, vals, params):
+    """VtkOutput (src/vtk.F90:15-77): <filename>.vti, text VTK ImageData with
+    one F30.10 value per line, x fastest.  vals[iz, iy, ix]."""
+    extent = "0 %5d 0 %5d 0 %5d" % (params.resx - 1, params.resy - 1, params.resz - 1)
+    lines = ['<?xml version="1.0"?>', '<VTKFile type="ImageData" version="0.1">',
+             '  <ImageData WholeExtent="%s" origin="0 0 0" spacing="1 1 1">' % extent,
+             '    <Piece Extent="%s">' % extent, '      <PointData Scalars="Result">',
+             '        <DataArray Name="Result" type="Float32">']
+    lines += ["          %.10f" % v for v in np.asarray(vals, dtype=np.float64).ravel()]
+    lines += ['        </DataArray>', '      </PointData>', '    </Piece>', '  </ImageData>', '</VTKFile>']
+    with open(str(filename) + ".vti", "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def PrintSolution(iter, t, ads, ads_data, directory="."):
+    """ADSS::PrintSolution (src/ADS.F90:955-989): step<iter>.vti with the
+    spline on the 31^3 grid of [0,1]^3."""
+    import os
+
+    params = PlotParams()
+    VtkOutput(os.path.join(directory, "step%d" % int(iter)), EvalGrid(ads_data, params), params)
 
 
 def Cleanup(ads, ads_data):

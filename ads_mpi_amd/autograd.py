@@ -50,11 +50,11 @@ class HeatSolver:
                    self.h)
         return out
 
-    def adjoint(self, c_n, lam_next, kappa, dt, dkappa=None, out=None):
+    def adjoint(self, c_n, lam_next, kappa, dt, dkappa=None, out=None, prec=_lib.ADS_PREC_FP64, refine=2):
         self._bind()
         out = torch.empty_like(lam_next) if out is None else out
-        _lib.check(self.L.ads_step_adjoint_d(self.h, _vp(c_n), _vp(lam_next), _vp(kappa), float(dt), _vp(out),
-                                             _vp(dkappa)), self.h)
+        _lib.check(self.L.ads_step_adjoint_d(self.h, _vp(c_n), _vp(lam_next), _vp(kappa), float(dt), int(prec),
+                                             int(refine), _vp(out), _vp(dkappa)), self.h)
         return out
 
     def close(self):
@@ -63,7 +63,7 @@ class HeatSolver:
 
 class HeatRollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, c0, kappa, solver, dt, steps, checkpoint_every):
+    def forward(ctx, c0, kappa, solver, dt, steps, checkpoint_every, prec, refine, adjoint_prec, adjoint_refine):
         k = max(1, int(checkpoint_every))
         c0d = c0.detach().contiguous()
         kd = kappa.detach().contiguous() if kappa is not None else None
@@ -75,9 +75,11 @@ class HeatRollout(torch.autograd.Function):
             if n % k == 0:
                 ckpts.append(c.clone())
             nxt = a if (c is not a) else b
-            solver.step(c, kd, dt, out=nxt)
+            solver.step(c, kd, dt, out=nxt, prec=prec, refine=refine)
             c = nxt
         ctx.solver = solver
+        ctx.prec, ctx.refine = int(prec), int(refine)
+        ctx.adjoint_prec, ctx.adjoint_refine = int(adjoint_prec), int(adjoint_refine)
         ctx.dt = float(dt)
         ctx.steps = int(steps)
         ctx.k = k
@@ -99,13 +101,21 @@ class HeatRollout(torch.autograd.Function):
             n1 = min(n0 + k, steps)
             states = [ctx.ckpts[s]]
             for n in range(n0, n1 - 1):
-                states.append(solver.step(states[-1], kd, dt))
+                states.append(solver.step(states[-1], kd, dt, prec=ctx.prec, refine=ctx.refine))
             for j in range(n1 - n0 - 1, -1, -1):
-                lam = solver.adjoint(states[j], lam, kd, dt, dkappa=dk, out=lam)
+                lam = solver.adjoint(states[j], lam, kd, dt, dkappa=dk, out=lam, prec=ctx.adjoint_prec,
+                                     refine=ctx.adjoint_refine)
         ctx.ckpts = None
-        return lam, dk, None, None, None, None
+        return lam, dk, None, None, None, None, None, None, None, None
 
 
-def heat_rollout(c0, kappa, solver, dt, steps, checkpoint_every=10):
-    """c_T after ``steps`` explicit heat steps, differentiable in c0 and kappa."""
-    return HeatRollout.apply(c0, kappa, solver, dt, steps, checkpoint_every)
+def heat_rollout(c0, kappa, solver, dt, steps, checkpoint_every=10, prec=_lib.ADS_PREC_FP64, refine=2,
+                 adjoint_prec=None, adjoint_refine=None):
+    """c_T after ``steps`` explicit heat steps, differentiable in c0 and kappa.
+
+    prec/refine select the forward form (ads_step_d); adjoint_prec and
+    adjoint_refine the adjoint form (ads_step_adjoint_d), by default the same.
+    The backward pass recomputes each segment with the forward form."""
+    ap = prec if adjoint_prec is None else adjoint_prec
+    ar = refine if adjoint_refine is None else adjoint_refine
+    return HeatRollout.apply(c0, kappa, solver, dt, steps, checkpoint_every, prec, refine, ap, ar)

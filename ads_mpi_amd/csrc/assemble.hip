@@ -119,8 +119,12 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
 {
     constexpr int KL = P - 1, W = kWave + KL, R = kRowsWg + KL, NT = P * (P + 1) / 2;
     constexpr int NTH = kWave * kRowsWg;
+    // The right-hand-side ring accumulates in FP64 also for T = float: on
+    // gfx950 ds_add_f32 issues about three times slower than ds_add_f64
+    // (LDS-active cycles 302M against 108M for the same instruction count).
+    using A = double;
     __shared__ T cs[P][R][W];  // coefficient planes, ring by z
-    __shared__ T fs[P][R][W];  // right-hand-side planes, ring by z
+    __shared__ A fs[P][R][W];  // right-hand-side planes, ring by z
     __shared__ T ns[GRAD ? P : 1][GRAD ? R : 1][GRAD ? W : 1];  // forward-state planes (GRAD)
     __shared__ double l2w[kRowsWg];
 
@@ -147,16 +151,34 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
     const T* __restrict__ Ex = emx + static_cast<size_t>(exc) * P * 2 * P;
     const T* __restrict__ Ey = emy + static_cast<size_t>(eyc) * 2 * NT;
 
-    for (int i = tid; i < P * R * W; i += NTH) (&fs[0][0][0])[i] = T(0);
+    for (int i = tid; i < P * R * W; i += NTH) (&fs[0][0][0])[i] = A(0);
 
-    auto load_plane = [&](int z) {
-        const int slot = z % P;
+    // plane z of c (and cn) is fetched into registers (NPT entries per
+    // thread) and written to its ring slot later, so the loads of the next
+    // plane overlap the current layer's work
+    constexpr int NE = R * W, NPT = (NE + NTH - 1) / NTH;
+    T pc[NPT], pn[GRAD ? NPT : 1];
+    auto fetch = [&](int z) {
         const long off = ex0 + static_cast<long>(sx) * ey0 + plane * z;
-        for (int i = tid; i < R * W; i += NTH) {
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * NTH;
             const int r = i / W, col = i - r * W;
-            const bool in = col < ncol && r < nrow;
-            cs[slot][r][col] = in ? T(c[off + col + static_cast<long>(sx) * r]) : T(0);
-            if constexpr (GRAD) ns[slot][r][col] = in ? T(cn[off + col + static_cast<long>(sx) * r]) : T(0);
+            const bool in = i < NE && col < ncol && r < nrow;
+            pc[k] = in ? T(c[off + col + static_cast<long>(sx) * r]) : T(0);
+            if constexpr (GRAD) pn[k] = in ? T(cn[off + col + static_cast<long>(sx) * r]) : T(0);
+        }
+    };
+    auto stash = [&](int z) {
+        const int slot = z % P;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * NTH;
+            if (i < NE) {
+                const int r = i / W, col = i - r * W;
+                cs[slot][r][col] = pc[k];
+                if constexpr (GRAD) ns[slot][r][col] = pn[k];
+            }
         }
     };
     // each entry is always handled by the same thread, so consecutive
@@ -166,18 +188,24 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
         T* dst = F + ex0 + static_cast<long>(sx) * ey0 + plane * z;
         for (int i = tid; i < R * W; i += NTH) {
             const int r = i / W, col = i - r * W;
-            if (col < ncol && r < nrow) atomic_add(&dst[col + static_cast<long>(sx) * r], fs[slot][r][col]);
-            fs[slot][r][col] = T(0);
+            if (col < ncol && r < nrow) atomic_add(&dst[col + static_cast<long>(sx) * r], T(fs[slot][r][col]));
+            fs[slot][r][col] = A(0);
         }
     };
 
-    for (int z = ez0; z < ez0 + KL; ++z) load_plane(z);
+    for (int z = ez0; z < ez0 + KL; ++z) {
+        fetch(z);
+        stash(z);
+    }
+    fetch(ez0 + KL);
 
     T l2 = T(0);
     for (int ez = ez0; ez < ez1; ++ez) {
         // plane ez+KL goes to the slot of plane ez-1, whose readers finished
-        // before the previous iteration's second barrier
-        load_plane(ez + KL);
+        // before the previous iteration's second barrier; then the loads of
+        // plane ez+KL+1 are issued
+        stash(ez + KL);
+        if (ez + 1 < ez1) fetch(ez + KL + 1);
         __syncthreads();
         if (row_ok) {
             const T* __restrict__ Ez = emz + static_cast<size_t>(ez) * 2 * NT;
@@ -248,7 +276,7 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
                         if constexpr (GRAD) gacc = fma(ns[slot[azo]][wv + ayo][lane + axo], ok, gacc);
                         // lanes of one wave address distinct columns here; the
                         // LDS applies one wave's adds in program order
-                        atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], fma(s, ok, om));
+                        atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], A(fma(s, ok, om)));
                     }
                     // keep the scheduler from computing every plane's values up front
                     __builtin_amdgcn_sched_barrier(0);
@@ -405,10 +433,10 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// z-segment length giving about `target` workgroups.
+// z-segment length giving at most `target` workgroups (at least one segment).
 int segment_length(long tiles_xy, int nez, long target)
 {
-    long nseg = (target + tiles_xy - 1) / tiles_xy;
+    long nseg = target / tiles_xy;
     if (nseg < 1) nseg = 1;
     if (nseg > nez) nseg = nez;
     return static_cast<int>((nez + nseg - 1) / nseg);
@@ -431,9 +459,13 @@ template <typename T, int P, bool GRAD>
 hipError_t heat_p(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
                   int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
 {
+    // two rounds of resident workgroups (a partly filled last round costs
+    // as much as a full one)
+    static const int resident =
+        resident_groups(reinterpret_cast<const void*>(&k_heat<T, P, GRAD>), kWave * kRowsWg, 0);
     AsmDims dm = base_dims(tb);
     dm.yblocks = (dm.ney + kRowsWg - 1) / kRowsWg;
-    dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.yblocks, dm.nez, 2048);
+    dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.yblocks, dm.nez, 2L * resident);
     const int nseg = (dm.nez + dm.zseg - 1) / dm.zseg;
     dim3 grid(dm.xchunks * dm.yblocks * nseg);
     *nparts = static_cast<int>(grid.x);
@@ -469,6 +501,37 @@ hipError_t assemble_p(int form, int f0, const double* c, T* F, const Tables<T>& 
                : project_p<T, P>(F, tb, f0, l2_part, part_cap, nparts, s);
 }
 
+// Output path (SaveSplinePlot + EvalSpline, src/plot.F90:97-145,
+// src/basis.F90:323-369): the spline at every point of a tensor grid, one
+// thread per point, from per-direction tables (first non-zero basis index
+// and the p+1 values).  The sum runs in EvalSpline's order (x outer, z
+// inner; b = bx*by*bz, then val += b*coeff) without fused multiply-adds.
+template <int P>
+__global__ __launch_bounds__(256) void k_eval_grid(const double* __restrict__ c, int sx, int sy,
+                                                   const int* __restrict__ fx, const double* __restrict__ bx, int rx,
+                                                   const int* __restrict__ fy, const double* __restrict__ by, int ry,
+                                                   const int* __restrict__ fz, const double* __restrict__ bz, int rz,
+                                                   double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    const long npts = static_cast<long>(rx) * ry * rz;
+    if (i >= npts) return;
+    const int ix = static_cast<int>(i % rx);
+    const int iy = static_cast<int>((i / rx) % ry);
+    const int iz = static_cast<int>(i / (static_cast<long>(rx) * ry));
+    const int x0 = fx[ix], y0 = fy[iy], z0 = fz[iz];
+    double val = 0.0;
+    for (int a = 0; a < P; ++a)
+        for (int b = 0; b < P; ++b)
+            for (int d = 0; d < P; ++d) {
+                const double w = bx[ix * P + a] * by[iy * P + b] * bz[iz * P + d];
+                const double cv = c[(x0 + a) + static_cast<long>(sx) * ((y0 + b) + static_cast<long>(sy) * (z0 + d))];
+                val = val + w * cv;
+            }
+    out[i] = val;
+}
+
 }  // namespace
 
 template <typename T>
@@ -489,25 +552,47 @@ template hipError_t launch_assemble<double>(int, int, int, const double*, double
 template hipError_t launch_assemble<float>(int, int, int, const double*, float*, const Tables<float>&, double,
                                            const double*, double*, int, int*, hipStream_t);
 
+hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, const int* fx, const double* bx, int rx,
+                            const int* fy, const double* by, int ry, const int* fz, const double* bz, int rz,
+                            double* out, hipStream_t s)
+{
+    const long npts = static_cast<long>(rx) * ry * rz;
+    const dim3 grid(static_cast<unsigned>((npts + 255) / 256));
+    switch (p) {
+    case 1: hipLaunchKernelGGL(k_eval_grid<2>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 2: hipLaunchKernelGGL(k_eval_grid<3>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 3: hipLaunchKernelGGL(k_eval_grid<4>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 4: hipLaunchKernelGGL(k_eval_grid<5>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, part, n, out);
     return hipGetLastError();
 }
 
-hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, double* lam_out, const Tables<double>& tb,
-                               double dt, const double* kappa, double* dkappa, double* l2_part, int part_cap,
-                               int* nparts, hipStream_t s)
+template <typename T>
+hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, T* lam_out, const Tables<T>& tb, double dt,
+                               const double* kappa, double* dkappa, double* l2_part, int part_cap, int* nparts,
+                               hipStream_t s)
 {
     if (!dkappa)
-        return launch_assemble<double>(p, kFormHeat, 0, mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, s);
+        return launch_assemble<T>(p, kFormHeat, 0, mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, s);
     switch (p) {
-    case 1: return heat_p<double, 2, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 2: return heat_p<double, 3, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 3: return heat_p<double, 4, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 4: return heat_p<double, 5, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 1: return heat_p<T, 2, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 2: return heat_p<T, 3, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 3: return heat_p<T, 4, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 4: return heat_p<T, 5, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
     default: return hipErrorInvalidValue;
     }
 }
+
+template hipError_t launch_heat_adjoint<double>(int, const double*, const double*, double*, const Tables<double>&,
+                                                double, const double*, double*, double*, int, int*, hipStream_t);
+template hipError_t launch_heat_adjoint<float>(int, const double*, const double*, float*, const Tables<float>&,
+                                               double, const double*, double*, double*, int, int*, hipStream_t);
 
 }  // namespace ads

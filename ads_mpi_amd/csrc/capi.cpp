@@ -65,6 +65,7 @@ struct ads_ctx {
     float* f32a = nullptr;
     float* f32b = nullptr;
     double* x64 = nullptr;
+    double* b64 = nullptr;
     double* mu = nullptr;
 
     std::vector<void*> allocs;
@@ -145,6 +146,8 @@ int upload_dir(ads_ctx* ctx, const ads::Direction& d, DevDir<T>& dd, ads::DirTab
     st.piv = dd.piv;
     st.bwd = dd.bwd;
     st.n = d.dofs;
+    st.pivots = 0;
+    for (int v : piv) st.pivots |= (v != 0);
     return ADS_OK;
 }
 
@@ -154,6 +157,7 @@ int ensure_ir_scratch(ads_ctx* ctx)
     if (!ctx->f32a && (rc = dev_alloc(ctx, &ctx->f32a, ctx->D))) return rc;
     if (!ctx->f32b && (rc = dev_alloc(ctx, &ctx->f32b, ctx->D))) return rc;
     if (!ctx->x64 && (rc = dev_alloc(ctx, &ctx->x64, ctx->D))) return rc;
+    if (!ctx->b64 && (rc = dev_alloc(ctx, &ctx->b64, ctx->D))) return rc;
     return ADS_OK;
 }
 
@@ -219,62 +223,112 @@ ads::BandSet band_set(const ads_ctx* ctx)
     return bs;
 }
 
-// c_out = M^-1 A c_in in FP32 + FP64-residual refinement (constant kappa).
-int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double kap, int refine, double* l2_d)
+// r = b + A c - M x in FP64 (nullptr operands are zero), rounded to FP32;
+// with l2_d also l2 = c^T M c -> l2_d.
+template <typename TX>
+int residual(ads_ctx* ctx, const double* c, const TX* x, const double* b, float* r, double dtk, double* l2_d)
+{
+    const ads::BandSet bs = band_set(ctx);
+    int nres = 0;
+    ADS_HIP(ctx, (ads::launch_kron_residual<TX, float>(ctx->p, c, x, b, r, bs, dtk, l2_d ? ctx->l2_part : nullptr,
+                                                       ctx->l2_cap, &nres, ctx->stream)));
+    if (l2_d) ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nres, l2_d, ctx->stream));
+    return ADS_OK;
+}
+
+// x = M^-1 b (M^-T b when trans) by FP32 sweeps corrected `corrections`
+// times with FP64 mass residuals r = b - M x (M is symmetric).  x0, when
+// given, is the FP32 first estimate; otherwise it is M^-1 float(b).  x must
+// not alias b.
+int mass_solve_ir(ads_ctx* ctx, const double* b, double* x, const float* x0, int corrections, bool trans)
+{
+    const float* d1 = x0;
+    if (!d1) {
+        ADS_HIP(ctx, ads::launch_convert_d2f(b, ctx->f32a, ctx->D, ctx->stream));
+        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32a, ctx->sw32, trans, ctx->stream));
+        d1 = ctx->f32a;
+    }
+    if (corrections == 0) {
+        ADS_HIP(ctx, ads::launch_add_f2d(d1, nullptr, x, ctx->D, ctx->stream));
+        return ADS_OK;
+    }
+    // the first correction reads the FP32 estimate directly
+    int rc = residual<float>(ctx, nullptr, d1, b, ctx->f32b, 0.0, nullptr);
+    if (rc) return rc;
+    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, trans, ctx->stream));
+    ADS_HIP(ctx, ads::launch_add_f2d(d1, ctx->f32b, x, ctx->D, ctx->stream));
+    for (int k = 1; k < corrections; ++k) {
+        if ((rc = residual<double>(ctx, nullptr, x, b, ctx->f32b, 0.0, nullptr))) return rc;
+        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, trans, ctx->stream));
+        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, x, ctx->D, ctx->stream));
+    }
+    return ADS_OK;
+}
+
+// c_out = M^-1 A(kappa) c_in: FP32 assembly and sweeps, then `refine`
+// corrections with FP64 residuals.  Constant kappa: the residual is the
+// Kronecker operator A c - M x (it also gives l2).  Per-element kappa: b =
+// A(kappa) c comes from the FP64 element assembly (with l2) and the
+// corrections use the mass residual b - M x.
+int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa_d, double dt, int refine,
+            double* l2_d)
 {
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
-    const ads::BandSet bs = band_set(ctx);
-    const double dtk = dt * kap;
-    // FP32 assembly (kappa folded into the constant) and FP32 sweeps
-    if ((rc = assemble<float>(ctx, ads::kFormHeat, 0, c_in, ctx->f32a, dtk, nullptr, l2_d))) return rc;
-    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32a, ctx->sw32, false, ctx->stream));
-    // first refinement: r = A c - M x0 (FP64), delta = M^-1 r (FP32); x
-    // accumulates in c_out unless the caller steps in place
+    // x accumulates in c_out unless the caller steps in place
     double* x = c_out != c_in ? c_out : ctx->x64;
-    ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, ctx->f32a, ctx->f32b, bs, dtk, nullptr,
-                                                          ctx->stream)));
-    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-    ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, ctx->f32b, x, ctx->D, ctx->stream));
-    for (int k = 1; k < refine; ++k) {
-        ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, x, ctx->f32b, bs, dtk, nullptr,
-                                                               ctx->stream)));
+    if ((rc = assemble<float>(ctx, ads::kFormHeat, 0, c_in, ctx->f32a, dt, kappa_d, nullptr))) return rc;
+    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32a, ctx->sw32, false, ctx->stream));
+    if (kappa_d) {
+        if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in, ctx->b64, dt, kappa_d, l2_d))) return rc;
+        if ((rc = mass_solve_ir(ctx, ctx->b64, x, ctx->f32a, refine, false))) return rc;
+    } else {
+        if ((rc = residual<float>(ctx, c_in, ctx->f32a, nullptr, ctx->f32b, dt, l2_d))) return rc;
         ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-        ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, x, ctx->D, ctx->stream));
+        ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, ctx->f32b, x, ctx->D, ctx->stream));
+        for (int k = 1; k < refine; ++k) {
+            if ((rc = residual<double>(ctx, c_in, x, nullptr, ctx->f32b, dt, nullptr))) return rc;
+            ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
+            ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, x, ctx->D, ctx->stream));
+        }
     }
     if (x != c_out)
         ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
     return ADS_OK;
 }
 
-// c_out = M^-1 A c_in from a zero predictor: each pass computes the FP64
-// Kronecker residual r = A c - M x and corrects x with FP32 sweeps.  The
-// first pass (x = 0) also gives l2norm = c^T M c, the Gauss integral of u^2.
-int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, double dt, double kap, int refine, double* l2_d)
+// c_out = M^-1 A(kappa) c_in from FP64 residuals and FP32 sweeps, `refine`
+// sweeps in all.  Constant kappa: each pass computes the Kronecker residual
+// r = A c - M x (the first, x = 0, also gives l2 = c^T M c); the first
+// correction d1 is FP32, so the second residual reads it as FP32 and
+// x = d1 + d2 is formed in FP64 once.  Per-element kappa: b = A(kappa) c
+// from the FP64 element assembly, then x = M^-1 b by mass_solve_ir.
+int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa_d, double dt, int refine,
+                 double* l2_d)
 {
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
-    const ads::BandSet bs = band_set(ctx);
-    const double dtk = dt * kap;
-    if (ads::kron_residual_partials(bs) > ctx->l2_cap)
-        return fail(ctx, ADS_ERR_INVALID, "internal: l2 partial buffer too small");
-    // x accumulates in c_out unless the caller steps in place
     double* x = c_out != c_in ? c_out : ctx->x64;
-    for (int k = 0; k < refine; ++k) {
-        if (k == 0) {
-            ADS_HIP(ctx, (ads::launch_kron_residual<float, float>(ctx->p, c_in, nullptr, ctx->f32b, bs, dtk,
-                                                                  l2_d ? ctx->l2_part : nullptr, ctx->stream)));
-            if (l2_d)
-                ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, ads::kron_residual_partials(bs), l2_d,
-                                                         ctx->stream));
+    if (kappa_d) {
+        if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in, ctx->b64, dt, kappa_d, l2_d))) return rc;
+        if ((rc = mass_solve_ir(ctx, ctx->b64, x, nullptr, refine - 1, false))) return rc;
+    } else {
+        float* d1 = ctx->f32a;
+        float* d2 = ctx->f32b;
+        if ((rc = residual<float>(ctx, c_in, nullptr, nullptr, d1, dt, l2_d))) return rc;
+        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, d1, ctx->sw32, false, ctx->stream));
+        if (refine == 1) {
+            ADS_HIP(ctx, ads::launch_add_f2d(d1, nullptr, x, ctx->D, ctx->stream));
         } else {
-            ADS_HIP(ctx, (ads::launch_kron_residual<double, float>(ctx->p, c_in, x, ctx->f32b, bs, dtk, nullptr,
-                                                                   ctx->stream)));
+            if ((rc = residual<float>(ctx, c_in, d1, nullptr, d2, dt, nullptr))) return rc;
+            ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, d2, ctx->sw32, false, ctx->stream));
+            ADS_HIP(ctx, ads::launch_add_f2d(d1, d2, x, ctx->D, ctx->stream));
+            for (int k = 2; k < refine; ++k) {
+                if ((rc = residual<double>(ctx, c_in, x, nullptr, d2, dt, nullptr))) return rc;
+                ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, d2, ctx->sw32, false, ctx->stream));
+                ADS_HIP(ctx, ads::launch_add_f2d(nullptr, d2, x, ctx->D, ctx->stream));
+            }
         }
-        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-        // x = delta on the first pass (no read of x), x += delta after
-        ADS_HIP(ctx, ads::launch_add_f2d(k == 0 ? ctx->f32b : nullptr, k == 0 ? nullptr : ctx->f32b, x, ctx->D,
-                                         ctx->stream));
     }
     if (x != c_out)
         ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
@@ -326,9 +380,9 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
     if (!rc) rc = dev_alloc(ctx, &ctx->F, ctx->D);
     {
         // partial sums: at most one per (64 x elements, y row, z layer) in the
-        // assembly, and kron_residual_partials in the Kronecker residual
+        // assembly, and one per (64 x DOFs, 4 y DOFs, z DOF) in the residual
         const long asm_cap = static_cast<long>((ctx->dir[0].nelem + 63) / 64) * ctx->dir[1].nelem * ctx->dir[2].nelem;
-        ctx->l2_cap = static_cast<int>(std::max<long>(asm_cap, ads::kron_residual_partials(band_set(ctx))));
+        ctx->l2_cap = static_cast<int>(std::max<long>(asm_cap, ads::kron_residual_partials_bound(band_set(ctx))));
     }
     if (!rc) rc = dev_alloc(ctx, &ctx->l2_part, ctx->l2_cap);
     if (!rc) rc = dev_alloc(ctx, &ctx->l2, 1);
@@ -518,37 +572,104 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
             ADS_HIP(ctx, hipMemcpyAsync(c_out_d, target, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
         return ADS_OK;
     }
-    if (prec == ADS_PREC_FP32_IR) {
+    if (prec == ADS_PREC_FP32_IR || prec == ADS_PREC_FP32_IR_KRON) {
         if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
-        if (kappa_d) return fail(ctx, ADS_ERR_UNSUPPORTED,
-                                 "the FP32+refinement form needs a constant diffusivity (its FP64 residual is the "
-                                 "Kronecker banded operator)");
-        return step_ir(ctx, c_in_d, c_out_d, dt, 1.0, refine, l2_d);
-    }
-    if (prec == ADS_PREC_FP32_IR_KRON) {
-        if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
-        if (kappa_d) return fail(ctx, ADS_ERR_UNSUPPORTED,
-                                 "the Kronecker refinement form needs a constant diffusivity");
-        return step_ir_kron(ctx, c_in_d, c_out_d, dt, 1.0, refine, l2_d);
+        if (c_in_d == c_out_d && c_out_d == ctx->x64)
+            return fail(ctx, ADS_ERR_INVALID, "internal: buffer aliasing");
+        return prec == ADS_PREC_FP32_IR ? step_ir(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d)
+                                        : step_ir_kron(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d);
     }
     return fail(ctx, ADS_ERR_INVALID, "unknown precision form");
 }
 
 int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next_d, const double* kappa_d, double dt,
-                       double* lam_out_d, double* dkappa_d)
+                       int prec, int refine, double* lam_out_d, double* dkappa_d)
 {
     if (check_ctx(ctx) || !c_n_d || !lam_next_d || !lam_out_d) return ADS_ERR_INVALID;
     if (lam_out_d == c_n_d) return fail(ctx, ADS_ERR_INVALID, "ads_step_adjoint_d: lam_out must not alias c_n");
     int rc;
     if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
-    // mu = M^-T lam_next
-    ADS_HIP(ctx, hipMemcpyAsync(ctx->mu, lam_next_d, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
-    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, true, ctx->stream));
-    // lam_out = A(kappa) mu (A symmetric) and dkappa += -dt c_n,e^T K_e mu_e in one pass
-    if ((rc = zero(ctx, lam_out_d, ctx->D))) return rc;
     int nparts = 0;
-    ADS_HIP(ctx, ads::launch_heat_adjoint(ctx->p, ctx->mu, c_n_d, lam_out_d, ctx->tb64, dt, kappa_d, dkappa_d,
-                                          ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream));
+    if (prec == ADS_PREC_FP64) {
+        // mu = M^-T lam_next
+        ADS_HIP(ctx, hipMemcpyAsync(ctx->mu, lam_next_d, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, true, ctx->stream));
+        // lam_out = A(kappa) mu (A symmetric) and dkappa += -dt c_n,e^T K_e mu_e in one pass
+        if ((rc = zero(ctx, lam_out_d, ctx->D))) return rc;
+        ADS_HIP(ctx, ads::launch_heat_adjoint<double>(ctx->p, ctx->mu, c_n_d, lam_out_d, ctx->tb64, dt, kappa_d,
+                                                      dkappa_d, ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream));
+        return ADS_OK;
+    }
+    if (prec != ADS_PREC_FP32_IR && prec != ADS_PREC_FP32_IR_KRON)
+        return fail(ctx, ADS_ERR_INVALID, "unknown precision form");
+    if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
+    if ((rc = ensure_ir_scratch(ctx))) return rc;
+    // mu = M^-T lam_next: FP32 transposed sweeps corrected with FP64 mass residuals
+    if ((rc = mass_solve_ir(ctx, lam_next_d, ctx->mu, nullptr, refine - 1, true))) return rc;
+    // lam_out = A(kappa) mu and dkappa in FP32 arithmetic (dkappa accumulates in FP64)
+    if ((rc = zero(ctx, ctx->f32a, ctx->D))) return rc;
+    ADS_HIP(ctx, ads::launch_heat_adjoint<float>(ctx->p, ctx->mu, c_n_d, ctx->f32a, ctx->tb32, dt, kappa_d, dkappa_d,
+                                                 ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream));
+    ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, nullptr, lam_out_d, ctx->D, ctx->stream));
+    return ADS_OK;
+}
+
+int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const double hi[3], double* vals)
+{
+    if (check_ctx(ctx) || !res || !lo || !hi || !vals) return ADS_ERR_INVALID;
+    long npts = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (res[d] < 2 || res[d] > 65536) return fail(ctx, ADS_ERR_INVALID, "ads_eval_grid: res must be 2..65536");
+        if (!std::isfinite(lo[d]) || !std::isfinite(hi[d])) return fail(ctx, ADS_ERR_INVALID, "ads_eval_grid: bad range");
+        npts *= res[d];
+    }
+    if (npts > (1L << 30)) return fail(ctx, ADS_ERR_INVALID, "ads_eval_grid: more than 2^30 points");
+    std::vector<int> first[3];
+    std::vector<double> tab[3];
+    for (int d = 0; d < 3; ++d) ads::grid_tables(ctx->dir[d], res[d], lo[d], hi[d], first[d], tab[d]);
+    // the tables and the values live only for this call
+    void* buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    auto release = [&]() {
+        for (void* b : buf)
+            if (b) (void)hipFree(b);
+    };
+    hipError_t e = hipSuccess;
+    for (int d = 0; d < 3 && e == hipSuccess; ++d) {
+        e = hipMalloc(&buf[2 * d], sizeof(int) * first[d].size());
+        if (e == hipSuccess) e = hipMalloc(&buf[2 * d + 1], sizeof(double) * tab[d].size());
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(buf[2 * d], first[d].data(), sizeof(int) * first[d].size(), hipMemcpyHostToDevice,
+                               ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(buf[2 * d + 1], tab[d].data(), sizeof(double) * tab[d].size(), hipMemcpyHostToDevice,
+                               ctx->stream);
+    }
+    if (e == hipSuccess) e = hipMalloc(&buf[6], sizeof(double) * npts);
+    if (e == hipSuccess)
+        e = ads::launch_eval_grid(ctx->p, ctx->c, ctx->dir[0].dofs, ctx->dir[1].dofs, static_cast<int*>(buf[0]),
+                                  static_cast<double*>(buf[1]), res[0], static_cast<int*>(buf[2]),
+                                  static_cast<double*>(buf[3]), res[1], static_cast<int*>(buf[4]),
+                                  static_cast<double*>(buf[5]), res[2], static_cast<double*>(buf[6]), ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(vals, buf[6], sizeof(double) * npts, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    release();
+    if (e != hipSuccess) return hip_fail(ctx, e, "ads_eval_grid");
+    return ADS_OK;
+}
+
+int ads_host_grid_tables(int n, int p, int res, double lo, double hi, int* first, double* vals)
+{
+    if (res < 2 || !first || !vals) return fail(nullptr, ADS_ERR_INVALID, "ads_host_grid_tables: bad arguments");
+    ads::Direction d;
+    std::string e = ads::build_direction(n, p, d);
+    if (!e.empty()) return fail(nullptr, e.find("singular") != std::string::npos ? ADS_ERR_SINGULAR : ADS_ERR_INVALID, e);
+    std::vector<int> f;
+    std::vector<double> v;
+    ads::grid_tables(d, res, lo, hi, f, v);
+    std::memcpy(first, f.data(), sizeof(int) * f.size());
+    std::memcpy(vals, v.data(), sizeof(double) * v.size());
     return ADS_OK;
 }
 
@@ -604,8 +725,8 @@ int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms
             e = ads::launch_assemble<double>(ctx->p, ads::kFormHeat, 0, ctx->c, ctx->mu, ctx->tb64, dt, ctx->kappa,
                                              ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream);
         else
-            e = ads::launch_kron_residual<float, float>(ctx->p, ctx->c, nullptr, ctx->f32b, bs, dt, ctx->l2_part,
-                                                        ctx->stream);
+            e = ads::launch_kron_residual<float, float>(ctx->p, ctx->c, nullptr, nullptr, ctx->f32b, bs, dt,
+                                                        ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream);
     }
     if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);

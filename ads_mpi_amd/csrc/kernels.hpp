@@ -33,6 +33,7 @@ struct SweepTables {
     const int* piv;  // [n]
     const T* bwd;    // [n][2p+1]
     int n;
+    int pivots;      // nonzero when the factorisation has row interchanges
 };
 
 enum Form { kFormHeat = 0, kFormProject = 1 };
@@ -50,10 +51,19 @@ hipError_t launch_assemble(int p, int form, int f0, const double* c, T* F, const
 
 // Adjoint assembly given mu = M^-T lam:  lam_out = A(kappa) mu (A symmetric)
 // and, when dkappa != nullptr, dkappa[e] += -dt c_n,e^T K_e mu_e in the same
-// pass.  lam_out must be zero on entry; l2_part as in launch_assemble.
-hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, double* lam_out, const Tables<double>& tb,
-                               double dt, const double* kappa, double* dkappa, double* l2_part, int part_cap,
-                               int* nparts, hipStream_t s);
+// pass (dkappa accumulates in FP64 for both T).  lam_out must be zero on
+// entry; l2_part as in launch_assemble.
+template <typename T>
+hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, T* lam_out, const Tables<T>& tb, double dt,
+                               const double* kappa, double* dkappa, double* l2_part, int part_cap, int* nparts,
+                               hipStream_t s);
+
+// The spline with coefficients c (x fastest, sx*sy per z plane) at the
+// tensor grid given by per-direction first-index and basis-value tables;
+// out x fastest (rx*ry*rz).
+hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, const int* fx, const double* bx, int rx,
+                            const int* fy, const double* by, int ry, const int* fz, const double* bz, int rz,
+                            double* out, hipStream_t s);
 
 // Deterministic sum of n partials into out[0].
 hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStream_t s);
@@ -63,22 +73,30 @@ hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStr
 template <typename T>
 hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans, hipStream_t s);
 
-// r = A c - M x in FP64 with the 1-D banded matrices (constant kappa):
+// r = b + A c - M x in FP64 with the 1-D banded matrices (constant kappa):
 // A = M - dt*kappa*K  where M = Mz(x)My(x)Mx, K = Mz(x)My(x)Kx + Mz(x)Ky(x)Mx + Kz(x)My(x)Mx.
-// x is read in precision TX (x == nullptr means x = 0); r is written in
-// precision TR.  With l2_part != nullptr each workgroup also writes its
-// partial sum of c^T M c; kron_residual_partials gives their count (for
-// launch_reduce_partials).
+// c, x and b may each be nullptr (zero).  x is read in precision TX; r is
+// written in precision TR.  With l2_part != nullptr each workgroup also
+// writes its partial sum of c^T M c; their count goes to *nparts (for
+// launch_reduce_partials), and a launch that would need more than part_cap
+// is refused before anything runs.
 struct BandSet {
     const double* mass[3];   // row tables: [i][t] = M(i, i-p+t), t = 0..2p, zero outside
     const double* stiff[3];  // [i][t] = K(i, i-p+t)
     int n[3];
 };
 template <typename TX, typename TR>
-hipError_t launch_kron_residual(int p, const double* c, const TX* x, TR* r, const BandSet& bs, double dtk,
-                                double* l2_part, hipStream_t s);
-int kron_residual_partials(const BandSet& bs);
+hipError_t launch_kron_residual(int p, const double* c, const TX* x, const double* b, TR* r, const BandSet& bs,
+                                double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s);
+// upper bound of the workgroup (partial-sum) count of launch_kron_residual
+long kron_residual_partials_bound(const BandSet& bs);
 
+// Workgroups of `fn` (threads each, lds dynamic bytes) resident at once on
+// the current device: occupancy per CU times the CU count.
+int resident_groups(const void* fn, int threads, size_t lds);
+
+// out = float(in)
+hipError_t launch_convert_d2f(const double* in, float* out, int64_t n, hipStream_t s);
 // out = (a ? a : out) + (b ? b : 0), FP32 operands widened to FP64
 hipError_t launch_add_f2d(const float* a, const float* b, double* out, int64_t n, hipStream_t s);
 

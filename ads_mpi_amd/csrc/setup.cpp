@@ -305,6 +305,40 @@ void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>
     }
 }
 
+int find_span(int n, int p, double u, const double* U)
+{
+    if (u >= U[n + 1]) return n;
+    if (u <= U[p]) return p;
+    int low = p, high = n + 1;
+    int mid = (low + high) / 2;
+    while (u < U[mid] || u >= U[mid + 1]) {
+        if (u < U[mid]) high = mid;
+        else low = mid;
+        mid = (low + high) / 2;
+    }
+    return mid;
+}
+
+void grid_tables(const Direction& d, int res, double lo, double hi, std::vector<int>& first,
+                 std::vector<double>& vals)
+{
+    const int P = d.p + 1;
+    first.assign(res, 0);
+    vals.assign(static_cast<size_t>(res) * P, 0.0);
+    double bv[(kMaxDegree + 1) * 2];
+    for (int i = 0; i < res; ++i) {
+        const double t = i / static_cast<double>(res - 1);
+        // (1 - t) * lo + t * hi, each product rounded (no fused multiply-add)
+        const double a = (1.0 - t) * lo;
+        const double b = t * hi;
+        const double u = a + b;
+        const int span = find_span(d.n, d.p, u, d.knots.data());
+        basis_and_derivative(span, u, d.p, d.knots.data(), bv);
+        first[i] = span - d.p;
+        for (int k = 0; k < P; ++k) vals[static_cast<size_t>(i) * P + k] = bv[k * 2];
+    }
+}
+
 uint64_t splitmix64(uint64_t x)
 {
     x += 0x9E3779B97F4A7C15ULL;

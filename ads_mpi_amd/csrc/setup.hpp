@@ -59,6 +59,17 @@ void dense_matrix(const Direction& d, const double mix[4], double* out);
 void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>& fwd_piv,
                   std::vector<double>& bwd);
 
+// FindSpan (A2.1, src/basis.F90:226-259): the knot span of u, with
+// u >= U(n+1) -> n and u <= U(p) -> p.
+int find_span(int n, int p, double u, const double* U);
+
+// Tables of the output path (SaveSplinePlot, src/plot.F90:97-145): for the
+// res points u_i = (1 - t) lo + t hi, t = i / (res - 1) (math.F90:44-51),
+// first[i] = span - p and vals[i*(p+1) + a] = the p+1 non-zero B-splines at
+// u_i (DersBasisFuns with d = 0, as EvalSpline, basis.F90:323-369).
+void grid_tables(const Direction& d, int res, double lo, double hi, std::vector<int>& first,
+                 std::vector<double>& vals);
+
 // Synthetic input generator shared with the host API (SplitMix64 counters).
 uint64_t splitmix64(uint64_t x);
 void fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);

@@ -27,7 +27,10 @@ constexpr int kWave = 64;
 //   TF  U^T solve (row form), ascending:              in(j)      -> out(j)
 //   TB  L^T solve + interchanges, descending:         in(j)      -> out(j+KL)
 // ---------------------------------------------------------------------------
-template <typename T, int KL>
+// PIV = false is the instance for factorisations without row interchanges
+// (every ipiv(j) = j, the case at all benchmark sizes): the swap is a no-op
+// there and the step is straight-line code.
+template <typename T, int KL, bool PIV>
 struct PassNF {
     static constexpr bool kAscending = true;
     static constexpr int kIn = KL + 1, kOut = 0;
@@ -40,15 +43,17 @@ struct PassNF {
     }
     __device__ T step(int j, T in, const SweepTables<T>& st)
     {
-        const int l = st.piv[j];
-        if (l != 0) {
+        if constexpr (PIV) {
+            const int l = st.piv[j];
+            if (l != 0) {
 #pragma unroll
-            for (int t = 1; t <= KL; ++t)
-                if (t == l) {
-                    const T tmp = w[0];
-                    w[0] = w[t];
-                    w[t] = tmp;
-                }
+                for (int t = 1; t <= KL; ++t)
+                    if (t == l) {
+                        const T tmp = w[0];
+                        w[0] = w[t];
+                        w[t] = tmp;
+                    }
+            }
         }
         const T xj = w[0];
 #pragma unroll
@@ -64,7 +69,7 @@ struct PassNF {
     }
 };
 
-template <typename T, int KL>
+template <typename T, int KL, bool PIV>
 struct PassNB {
     static constexpr bool kAscending = false;
     static constexpr int K = 2 * KL;
@@ -93,7 +98,7 @@ struct PassNB {
     }
 };
 
-template <typename T, int KL>
+template <typename T, int KL, bool PIV>
 struct PassTF {
     static constexpr bool kAscending = true;
     static constexpr int K = 2 * KL;
@@ -123,7 +128,7 @@ struct PassTF {
     }
 };
 
-template <typename T, int KL>
+template <typename T, int KL, bool PIV>
 struct PassTB {
     static constexpr bool kAscending = false;
     static constexpr int kIn = 0, kOut = KL;
@@ -142,15 +147,17 @@ struct PassTB {
 #pragma unroll
         for (int t = 1; t <= KL; ++t) s += w[t] * st.fwd[j * KL + t - 1];
         w[0] = in - s;
-        const int l = st.piv[j];
-        if (l != 0) {
+        if constexpr (PIV) {
+            const int l = st.piv[j];
+            if (l != 0) {
 #pragma unroll
-            for (int t = 1; t <= KL; ++t)
-                if (t == l) {
-                    const T tmp = w[0];
-                    w[0] = w[t];
-                    w[t] = tmp;
-                }
+                for (int t = 1; t <= KL; ++t)
+                    if (t == l) {
+                        const T tmp = w[0];
+                        w[0] = w[t];
+                        w[t] = tmp;
+                    }
+            }
         }
         return w[KL];  // b(j + KL) is final once step j is done
     }
@@ -188,38 +195,72 @@ __device__ __forceinline__ SweepTables<T> stage_tables(const SweepTables<T>& g, 
     t.bwd = bwd;
     t.piv = piv;
     t.n = n;
+    t.pivots = g.pivots;
     return t;
+}
+
+// Steps j of chunk ch (CH steps) in the pass's direction.
+template <class PS, int CH>
+__device__ __forceinline__ int chunk_step(int ch, int i, int n)
+{
+    return PS::kAscending ? ch * CH + i : n - 1 - (ch * CH + i);
+}
+
+// True when every step of chunk ch and its element j + off lie inside the
+// line: such chunks run without per-step bounds checks.
+template <class PS, int CH>
+__device__ __forceinline__ bool chunk_inside(int ch, int nch, int n, int off)
+{
+    if (ch >= nch) return false;
+    const int a = chunk_step<PS, CH>(ch, 0, n), b = chunk_step<PS, CH>(ch, CH - 1, n);
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo >= 0 && hi < n && lo + off >= 0 && hi + off < n;
 }
 
 // One lane per line; element k of the line at base[k*stride].  Inputs are
 // fetched CH at a time one chunk ahead of the recurrence (double buffer in
 // VGPRs), so every lane keeps CH loads in flight.
-template <typename T, int KL, template <typename, int> class PASS, int CH>
+template <typename T, int KL, template <typename, int, bool> class PASS, int CH, bool PIV>
 __device__ __forceinline__ void run_line(T* __restrict__ base, long stride, int n, const SweepTables<T>& st)
 {
-    using PS = PASS<T, KL>;
+    using PS = PASS<T, KL, PIV>;
     PS ps;
     ps.prologue([&](int k) { return base[static_cast<long>(k) * stride]; }, n);
     const int nch = (n + CH - 1) / CH;
     auto fetch = [&](int ch, T* buf) {
+        if (chunk_inside<PS, CH>(ch, nch, n, PS::kIn)) {
 #pragma unroll
-        for (int i = 0; i < CH; ++i) {
-            const int j = PS::kAscending ? ch * CH + i : n - 1 - (ch * CH + i);
-            const int k = j + PS::kIn;
-            buf[i] = (ch < nch && j >= 0 && j < n && k >= 0 && k < n) ? base[static_cast<long>(k) * stride] : T(0);
+            for (int i = 0; i < CH; ++i)
+                buf[i] = base[static_cast<long>(chunk_step<PS, CH>(ch, i, n) + PS::kIn) * stride];
+        } else {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int j = chunk_step<PS, CH>(ch, i, n);
+                const int k = j + PS::kIn;
+                buf[i] = (ch < nch && j >= 0 && j < n && k >= 0 && k < n) ? base[static_cast<long>(k) * stride]
+                                                                         : T(0);
+            }
         }
     };
     T cur[CH], nxt[CH];
     fetch(0, cur);
     for (int ch = 0; ch < nch; ++ch) {
         fetch(ch + 1, nxt);
+        if (chunk_inside<PS, CH>(ch, nch, n, PS::kOut)) {
 #pragma unroll
-        for (int i = 0; i < CH; ++i) {
-            const int j = PS::kAscending ? ch * CH + i : n - 1 - (ch * CH + i);
-            if (j >= 0 && j < n) {
-                const T o = ps.step(j, cur[i], st);
-                const int k = j + PS::kOut;
-                if (k < n) base[static_cast<long>(k) * stride] = o;
+            for (int i = 0; i < CH; ++i) {
+                const int j = chunk_step<PS, CH>(ch, i, n);
+                base[static_cast<long>(j + PS::kOut) * stride] = ps.step(j, cur[i], st);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int j = chunk_step<PS, CH>(ch, i, n);
+                if (j >= 0 && j < n) {
+                    const T o = ps.step(j, cur[i], st);
+                    const int k = j + PS::kOut;
+                    if (k < n) base[static_cast<long>(k) * stride] = o;
+                }
             }
         }
 #pragma unroll
@@ -230,14 +271,14 @@ __device__ __forceinline__ void run_line(T* __restrict__ base, long stride, int 
 
 template <typename T>
 struct LineChunk {
-    static constexpr int value = sizeof(T) == 8 ? 32 : 48;
+    static constexpr int value = sizeof(T) == 8 ? 32 : 64;
 };
 
 // Lines along a strided axis, both passes of the solve in one launch (the
 // second pass re-reads what the lane itself just wrote, partly from cache).
 // Line L = (i, j) with i in [0, ni) fastest: base = data + i + j*stride_j;
 // line element k at base + k*stride_k.
-template <typename T, int KL, bool TRANS>
+template <typename T, int KL, bool TRANS, bool PIV>
 __global__ __launch_bounds__(256) void k_sweep_lines(T* __restrict__ data, int ni, long stride_j, long stride_k,
                                                      int nlines, SweepTables<T> gst)
 {
@@ -249,26 +290,38 @@ __global__ __launch_bounds__(256) void k_sweep_lines(T* __restrict__ data, int n
     T* base = data + i + static_cast<long>(j) * stride_j;
     constexpr int CH = LineChunk<T>::value;
     if constexpr (!TRANS) {
-        run_line<T, KL, PassNF, CH>(base, stride_k, st.n, st);
-        run_line<T, KL, PassNB, CH>(base, stride_k, st.n, st);
+        run_line<T, KL, PassNF, CH, PIV>(base, stride_k, st.n, st);
+        run_line<T, KL, PassNB, CH, PIV>(base, stride_k, st.n, st);
     } else {
-        run_line<T, KL, PassTF, CH>(base, stride_k, st.n, st);
-        run_line<T, KL, PassTB, CH>(base, stride_k, st.n, st);
+        run_line<T, KL, PassTF, CH, PIV>(base, stride_k, st.n, st);
+        run_line<T, KL, PassTB, CH, PIV>(base, stride_k, st.n, st);
     }
 }
 
-// Contiguous rows (the x direction): a wave owns 64 rows and moves them
-// through LDS in CH-wide tiles so every global access is row-contiguous; the
-// next tile is fetched into VGPRs while the current one is solved.
-template <typename T, int KL, template <typename, int> class PASS>
-__device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int row0, int lane,
-                                         const SweepTables<T>& st, T* tin, T* tout)
+// LDS ordering between the lanes of one wave (tiles are private to a wave).
+__device__ __forceinline__ void wave_lds_sync()
 {
-    constexpr int CH = 16;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kRowTile = 16;   // columns per LDS tile
+constexpr int kRowWaves = 4;   // at most this many waves per row-sweep workgroup (one table copy)
+
+// Contiguous rows (the x direction): a wave owns 64 rows and moves them
+// through its own LDS tile, kRowTile columns at a time, so every global
+// access is row-contiguous; the next tile is fetched into VGPRs while the
+// current one is solved in place.
+template <typename T, int KL, template <typename, int, bool> class PASS, bool PIV>
+__device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int row0, int lane,
+                                         const SweepTables<T>& st, T* tile)
+{
+    constexpr int CH = kRowTile;
     constexpr int LDT = CH + 1;
     constexpr int RPI = kWave / CH;      // rows per cooperative instruction
     constexpr int NIT = kWave / RPI;     // cooperative instructions per tile
-    using PS = PASS<T, KL>;
+    using PS = PASS<T, KL, PIV>;
     const int n = st.n;
     const long rowlen = n;
     const int myrow = row0 + lane;
@@ -298,15 +351,23 @@ __device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int ro
         fetch(ch + 1, nxt);
         const int jlo = jlo_of(ch);
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) tin[(it * RPI + rsub) * LDT + cidx] = cur[it];
-        __syncthreads();
+        for (int it = 0; it < NIT; ++it) tile[(it * RPI + rsub) * LDT + cidx] = cur[it];
+        wave_lds_sync();
+        if (jlo >= 0 && jlo + CH <= n) {
 #pragma unroll
-        for (int i = 0; i < CH; ++i) {
-            const int c = PS::kAscending ? i : CH - 1 - i;
-            const int j = jlo + c;
-            if (j >= 0 && j < n) tout[lane * LDT + c] = ps.step(j, tin[lane * LDT + c], st);
+            for (int i = 0; i < CH; ++i) {
+                const int c = PS::kAscending ? i : CH - 1 - i;
+                tile[lane * LDT + c] = ps.step(jlo + c, tile[lane * LDT + c], st);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const int c = PS::kAscending ? i : CH - 1 - i;
+                const int j = jlo + c;
+                if (j >= 0 && j < n) tile[lane * LDT + c] = ps.step(j, tile[lane * LDT + c], st);
+            }
         }
-        __syncthreads();
+        wave_lds_sync();
         {
             const int j = jlo + cidx;
             const int k = j + PS::kOut;
@@ -314,38 +375,61 @@ __device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int ro
             for (int it = 0; it < NIT; ++it) {
                 const int grow = row0 + it * RPI + rsub;
                 if (grow < nrows && j >= 0 && j < n && k < n)
-                    data[static_cast<long>(grow) * rowlen + k] = tout[(it * RPI + rsub) * LDT + cidx];
+                    data[static_cast<long>(grow) * rowlen + k] = tile[(it * RPI + rsub) * LDT + cidx];
             }
         }
+        // the next tile overwrites this one: its reads above come first
+        wave_lds_sync();
 #pragma unroll
         for (int it = 0; it < NIT; ++it) cur[it] = nxt[it];
     }
     if (active) ps.epilogue([&](int k, T v) { mine[k] = v; }, n);
 }
 
-template <typename T, int KL, bool TRANS>
-__global__ __launch_bounds__(kWave) void k_sweep_rows(T* __restrict__ data, int nrows, SweepTables<T> gst)
+template <typename T, int KL>
+constexpr size_t row_table_bytes(int n)
 {
-    constexpr int LDT = 16 + 1;
-    __shared__ T tin[kWave * LDT];
-    __shared__ T tout[kWave * LDT];
+    return (sweep_table_bytes<T, KL>(n) + 15) & ~static_cast<size_t>(15);
+}
+
+constexpr size_t kLdsPerGroup = 65536;
+
+// waves per row-sweep workgroup: as many tiles as fit beside the tables
+template <typename T, int KL>
+int row_waves(int n)
+{
+    const size_t tile = static_cast<size_t>(kWave) * (kRowTile + 1) * sizeof(T);
+    const size_t tb = row_table_bytes<T, KL>(n);
+    if (tb + tile > kLdsPerGroup) return 0;
+    const size_t w = (kLdsPerGroup - tb) / tile;
+    return static_cast<int>(w < static_cast<size_t>(kRowWaves) ? w : kRowWaves);
+}
+
+template <typename T, int KL, bool TRANS, bool PIV>
+__global__ __launch_bounds__(kWave* kRowWaves) void k_sweep_rows(T* __restrict__ data, int nrows,
+                                                                 SweepTables<T> gst)
+{
     extern __shared__ unsigned char smem[];
     const SweepTables<T> st = stage_tables<T, KL>(gst, smem);
-    const int lane = threadIdx.x;
-    const int row0 = blockIdx.x * kWave;
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    // each wave's tile follows the tables
+    T* tile = reinterpret_cast<T*>(smem + row_table_bytes<T, KL>(gst.n)) + static_cast<size_t>(wv) * kWave * (kRowTile + 1);
+    const int row0 = (blockIdx.x * nw + wv) * kWave;
+    if (row0 >= nrows) return;
     if constexpr (!TRANS) {
-        run_rows<T, KL, PassNF>(data, nrows, row0, lane, st, tin, tout);
-        __syncthreads();
-        run_rows<T, KL, PassNB>(data, nrows, row0, lane, st, tin, tout);
+        run_rows<T, KL, PassNF, PIV>(data, nrows, row0, lane, st, tile);
+        run_rows<T, KL, PassNB, PIV>(data, nrows, row0, lane, st, tile);
     } else {
-        run_rows<T, KL, PassTF>(data, nrows, row0, lane, st, tin, tout);
-        __syncthreads();
-        run_rows<T, KL, PassTB>(data, nrows, row0, lane, st, tin, tout);
+        run_rows<T, KL, PassTF, PIV>(data, nrows, row0, lane, st, tile);
+        run_rows<T, KL, PassTB, PIV>(data, nrows, row0, lane, st, tile);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Kronecker residual r = A c - M x, A = M - dtk K (constant kappa), in FP64:
+// Kronecker residual r = b + A c - M x, A = M - dtk K (constant kappa), in
+// FP64 (c, x and b optional: nullptr means zero; with c = 0 and b = A(kappa)c
+// from the element assembly it is the mass residual b - M x of a variable
+// diffusivity):
 //   X: e = Mx(c - x) - dtk Kx c,   m = Mx c
 //   Y: f = My e - dtk Ky m,        g = My m
 //   Z: r = Mz f - dtk Kz g,        and M c = Mz g for l2 = c^T M c
@@ -375,7 +459,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 
 template <int P, typename TX, typename TR>
 __global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
-    const double* __restrict__ c, const TX* __restrict__ x, TR* __restrict__ r, const double* __restrict__ mx_rows,
+    const double* __restrict__ c, const TX* __restrict__ x, const double* __restrict__ b, TR* __restrict__ r,
+    const double* __restrict__ mx_rows,
     const double* __restrict__ kx_rows, const double* __restrict__ my_rows, const double* __restrict__ ky_rows,
     const double* __restrict__ mz_rows, const double* __restrict__ kz_rows, double dtk, ResDims dm,
     double* __restrict__ l2_part)
@@ -419,19 +504,40 @@ __global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
     for (int t = 0; t <= KL; ++t) cw[t] = 0.0;
     double l2 = 0.0;
 
-    for (int zi = zs - KL; zi < ze + KL; ++zi) {
+    // the tiles of the next two input planes are fetched into registers
+    // while the current one is processed (NPT entries per thread each)
+    constexpr int NT = TYH * TXH, NPT = (NT + NTH - 1) / NTH;
+    const int zend = ze + KL;
+    auto fetch = [&](int zi, double* pc, double* px) {
         const bool zok = zi >= 0 && zi < dm.nz;
-        for (int i = tid; i < TYH * TXH; i += NTH) {
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * NTH;
             const int rr = i / TXH, cc = i - rr * TXH;
             const int yy = y0 - KL + rr, xx = x0 - KL + cc;
-            const bool ok = zok && yy >= 0 && yy < dm.ny && xx >= 0 && xx < dm.nx;
+            const bool ok = i < NT && zok && yy >= 0 && yy < dm.ny && xx >= 0 && xx < dm.nx;
             const long at = xx + static_cast<long>(dm.nx) * yy + sxy * zi;
-            const double cv = ok ? c[at] : 0.0;
-            double dv = cv;
-            if (x) dv = ok ? cv - static_cast<double>(x[at]) : 0.0;
-            tc[rr][cc] = cv;
-            td[rr][cc] = dv;
+            pc[k] = (c && ok) ? c[at] : 0.0;
+            px[k] = (x && ok) ? static_cast<double>(x[at]) : 0.0;
         }
+    };
+    auto stash = [&](const double* pc, const double* px) {
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * NTH;
+            if (i < NT) {
+                const int rr = i / TXH, cc = i - rr * TXH;
+                tc[rr][cc] = pc[k];
+                td[rr][cc] = pc[k] - px[k];
+            }
+        }
+    };
+
+    auto plane = [&](int zi, double* pc, double* px) {
+        // this plane's tile goes to LDS (the previous plane's readers finished
+        // before the last barrier), then the loads of plane zi+2 are issued
+        stash(pc, px);
+        if (zi + 2 < zend) fetch(zi + 2, pc, px);
         __syncthreads();
         for (int rr = wv; rr < TYH; rr += kResRows) {
             double se = 0.0, sm = 0.0, sk = 0.0;
@@ -477,7 +583,11 @@ __global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
                 ok = fma(kzr[t], gw[t], ok);
             }
             o = fma(-dtk, ok, o);
-            if (xin && yin) r[gx + static_cast<long>(dm.nx) * gy + sxy * zo] = static_cast<TR>(o);
+            if (xin && yin) {
+                const long at = gx + static_cast<long>(dm.nx) * gy + sxy * zo;
+                if (b) o += b[at];
+                r[at] = static_cast<TR>(o);
+            }
             if (l2_part) {
                 double mc = 0.0;
 #pragma unroll
@@ -485,6 +595,14 @@ __global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
                 l2 = fma(cw[0], mc, l2);
             }
         }
+    };
+
+    double pa[NPT], xa[NPT], pb[NPT], xb[NPT];
+    fetch(zs - KL, pa, xa);
+    fetch(zs - KL + 1, pb, xb);
+    for (int zi = zs - KL; zi < zend; zi += 2) {
+        plane(zi, pa, xa);
+        if (zi + 1 < zend) plane(zi + 1, pb, xb);
     }
     if (l2_part) {
 #pragma unroll
@@ -509,31 +627,43 @@ __global__ __launch_bounds__(256) void k_add_f2d(const float* __restrict__ a, co
     out[i] = b ? base + static_cast<double>(b[i]) : base;
 }
 
-template <typename T, int KL>
-hipError_t kron_solve_p(T* F, const SweepTables<T> sw[3], bool trans, hipStream_t s)
+template <typename T, int KL, bool TRANS, bool PIV>
+void kron_solve_launch(T* F, const SweepTables<T> sw[3], int rw, size_t bx, size_t by, size_t bz, hipStream_t s)
 {
     const int nx = sw[0].n, ny = sw[1].n, nz = sw[2].n;
     const long sxy = static_cast<long>(nx) * ny;
     const int xrows = ny * nz, ylines = nx * nz, zlines = nx * ny;
-    const size_t bx = sweep_table_bytes<T, KL>(nx), by = sweep_table_bytes<T, KL>(ny), bz = sweep_table_bytes<T, KL>(nz);
-    // LDS budget: rows kernel = 17.4 KB of tiles + tables, lines kernel = tables (64 KB per block)
-    if (bx > 47104 || by > 61440 || bz > 61440) return hipErrorInvalidValue;  // line too long for the LDS tables
-    dim3 gx((xrows + kWave - 1) / kWave), gy((ylines + 255) / 256), gz((zlines + 255) / 256);
+    const int rows_per_block = kWave * rw;
+    dim3 gx((xrows + rows_per_block - 1) / rows_per_block), gy((ylines + 255) / 256), gz((zlines + 255) / 256);
+    hipLaunchKernelGGL((k_sweep_rows<T, KL, TRANS, PIV>), gx, dim3(kWave * rw), bx, s, F, xrows, sw[0]);
+    hipLaunchKernelGGL((k_sweep_lines<T, KL, TRANS, PIV>), gy, dim3(256), by, s, F, nx, sxy, static_cast<long>(nx),
+                       ylines, sw[1]);
+    hipLaunchKernelGGL((k_sweep_lines<T, KL, TRANS, PIV>), gz, dim3(256), bz, s, F, zlines, 0L, sxy, zlines, sw[2]);
+}
+
+template <typename T, int KL>
+hipError_t kron_solve_p(T* F, const SweepTables<T> sw[3], bool trans, hipStream_t s)
+{
+    const int nx = sw[0].n, ny = sw[1].n, nz = sw[2].n;
+    const int rw = row_waves<T, KL>(nx);
+    const size_t bx = row_table_bytes<T, KL>(nx) + static_cast<size_t>(rw) * kWave * (kRowTile + 1) * sizeof(T);
+    const size_t by = sweep_table_bytes<T, KL>(ny), bz = sweep_table_bytes<T, KL>(nz);
+    if (rw < 1 || by > kLdsPerGroup || bz > kLdsPerGroup) return hipErrorInvalidValue;  // line too long for the LDS tables
+    // the pivot-free instance when no direction's factorisation has interchanges
+    const bool piv = sw[0].pivots || sw[1].pivots || sw[2].pivots;
     if (!trans) {
-        hipLaunchKernelGGL((k_sweep_rows<T, KL, false>), gx, dim3(kWave), bx, s, F, xrows, sw[0]);
-        hipLaunchKernelGGL((k_sweep_lines<T, KL, false>), gy, dim3(256), by, s, F, nx, sxy, static_cast<long>(nx),
-                           ylines, sw[1]);
-        hipLaunchKernelGGL((k_sweep_lines<T, KL, false>), gz, dim3(256), bz, s, F, zlines, 0L, sxy, zlines, sw[2]);
+        if (!piv) kron_solve_launch<T, KL, false, false>(F, sw, rw, bx, by, bz, s);
+        else kron_solve_launch<T, KL, false, true>(F, sw, rw, bx, by, bz, s);
     } else {
-        hipLaunchKernelGGL((k_sweep_rows<T, KL, true>), gx, dim3(kWave), bx, s, F, xrows, sw[0]);
-        hipLaunchKernelGGL((k_sweep_lines<T, KL, true>), gy, dim3(256), by, s, F, nx, sxy, static_cast<long>(nx),
-                           ylines, sw[1]);
-        hipLaunchKernelGGL((k_sweep_lines<T, KL, true>), gz, dim3(256), bz, s, F, zlines, 0L, sxy, zlines, sw[2]);
+        if (!piv) kron_solve_launch<T, KL, true, false>(F, sw, rw, bx, by, bz, s);
+        else kron_solve_launch<T, KL, true, true>(F, sw, rw, bx, by, bz, s);
     }
     return hipGetLastError();
 }
 
-int residual_blocks(const BandSet& bs, ResDims* dm_out)
+// The z segments are sized so that one round of resident workgroups covers
+// the grid (a partly filled second round would double the time).
+ResDims residual_dims(const BandSet& bs, long resident)
 {
     ResDims dm{};
     dm.nx = bs.n[0];
@@ -542,22 +672,24 @@ int residual_blocks(const BandSet& bs, ResDims* dm_out)
     dm.xchunks = (dm.nx + kWave - 1) / kWave;
     dm.yblocks = (dm.ny + kResRows - 1) / kResRows;
     const long tiles = static_cast<long>(dm.xchunks) * dm.yblocks;
-    long nseg = (1024 + tiles - 1) / tiles;
+    long nseg = resident / tiles;
     if (nseg > dm.nz) nseg = dm.nz;
     if (nseg < 1) nseg = 1;
     dm.zseg = static_cast<int>((dm.nz + nseg - 1) / nseg);
-    nseg = (dm.nz + dm.zseg - 1) / dm.zseg;
-    if (dm_out) *dm_out = dm;
-    return static_cast<int>(tiles * nseg);
+    return dm;
 }
 
 template <int P, typename TX, typename TR>
-hipError_t residual_p(const double* c, const TX* x, TR* r, const BandSet& bs, double dtk, double* l2_part,
-                      hipStream_t s)
+hipError_t residual_p(const double* c, const TX* x, const double* b, TR* r, const BandSet& bs, double dtk,
+                      double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
-    ResDims dm;
-    const int blocks = residual_blocks(bs, &dm);
-    hipLaunchKernelGGL((k_kron_residual<P, TX, TR>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, r,
+    static const int resident =
+        resident_groups(reinterpret_cast<const void*>(&k_kron_residual<P, TX, TR>), kWave * kResRows, 0);
+    const ResDims dm = residual_dims(bs, resident);
+    const int blocks = dm.xchunks * dm.yblocks * ((dm.nz + dm.zseg - 1) / dm.zseg);
+    *nparts = blocks;
+    if (l2_part && blocks > part_cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_kron_residual<P, TX, TR>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, b, r,
                        bs.mass[0], bs.stiff[0], bs.mass[1], bs.stiff[1], bs.mass[2], bs.stiff[2], dtk, dm, l2_part);
     return hipGetLastError();
 }
@@ -579,29 +711,56 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
 template hipError_t launch_kron_solve<double>(int, double*, const SweepTables<double>[3], bool, hipStream_t);
 template hipError_t launch_kron_solve<float>(int, float*, const SweepTables<float>[3], bool, hipStream_t);
 
-int kron_residual_partials(const BandSet& bs) { return residual_blocks(bs, nullptr); }
+long kron_residual_partials_bound(const BandSet& bs)
+{
+    return static_cast<long>((bs.n[0] + kWave - 1) / kWave) * ((bs.n[1] + kResRows - 1) / kResRows) * bs.n[2];
+}
+
+int resident_groups(const void* fn, int threads, size_t lds)
+{
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    return per_cu * cus;
+}
 
 template <typename TX, typename TR>
-hipError_t launch_kron_residual(int p, const double* c, const TX* x, TR* r, const BandSet& bs, double dtk,
-                                double* l2_part, hipStream_t s)
+hipError_t launch_kron_residual(int p, const double* c, const TX* x, const double* b, TR* r, const BandSet& bs,
+                                double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
     switch (p) {
-    case 1: return residual_p<2, TX, TR>(c, x, r, bs, dtk, l2_part, s);
-    case 2: return residual_p<3, TX, TR>(c, x, r, bs, dtk, l2_part, s);
-    case 3: return residual_p<4, TX, TR>(c, x, r, bs, dtk, l2_part, s);
-    case 4: return residual_p<5, TX, TR>(c, x, r, bs, dtk, l2_part, s);
+    case 1: return residual_p<2, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
+    case 2: return residual_p<3, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
+    case 3: return residual_p<4, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
+    case 4: return residual_p<5, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_kron_residual<float, float>(int, const double*, const float*, float*, const BandSet&,
-                                                       double, double*, hipStream_t);
-template hipError_t launch_kron_residual<double, float>(int, const double*, const double*, float*, const BandSet&,
-                                                        double, double*, hipStream_t);
-template hipError_t launch_kron_residual<double, double>(int, const double*, const double*, double*,
-                                                         const BandSet&, double, double*, hipStream_t);
-template hipError_t launch_kron_residual<float, double>(int, const double*, const float*, double*, const BandSet&,
-                                                        double, double*, hipStream_t);
+template hipError_t launch_kron_residual<float, float>(int, const double*, const float*, const double*, float*,
+                                                       const BandSet&, double, double*, int, int*, hipStream_t);
+template hipError_t launch_kron_residual<double, float>(int, const double*, const double*, const double*, float*,
+                                                        const BandSet&, double, double*, int, int*, hipStream_t);
+template hipError_t launch_kron_residual<double, double>(int, const double*, const double*, const double*, double*,
+                                                         const BandSet&, double, double*, int, int*, hipStream_t);
+template hipError_t launch_kron_residual<float, double>(int, const double*, const float*, const double*, double*,
+                                                        const BandSet&, double, double*, int, int*, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_convert_d2f(const double* __restrict__ in, float* __restrict__ out, long n)
+{
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    if (i < n) out[i] = static_cast<float>(in[i]);
+}
+
+hipError_t launch_convert_d2f(const double* in, float* out, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_convert_d2f, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, in, out,
+                       static_cast<long>(n));
+    return hipGetLastError();
+}
 
 hipError_t launch_add_f2d(const float* a, const float* b, double* out, int64_t n, hipStream_t s)
 {

@@ -227,15 +227,33 @@ contains
       write(*, '(a)') 'MultiStep: mix-based alternating-direction substeps are not implemented'
    end subroutine MultiStep
 
-   ! ADSS::PrintSolution (src/ADS.F90:955-989): the VTK writer is outside the
-   ! hot path; this refreshes ads_data%F so a caller can write it.
+   ! ADSS::PrintSolution (src/ADS.F90:955-989): refreshes ads_data%F and
+   ! writes step<iter>.vti, the spline sampled on the 31^3 grid of [0,1]^3
+   ! (PlotParams(0,1,0,1,0,1,31,31,31), EvalSpline on the GPU, VtkOutput).
    subroutine PrintSolution(iter, t, ads, ads_data)
+      use plot, only: PlotParams
+      use vtk, only: VtkOutput
       integer(kind=4), intent(in) :: iter
       real(kind=8), intent(in) :: t
       type(ADS_setup), intent(in) :: ads
       type(ADS_compute_data), intent(inout) :: ads_data
       integer(kind=4) :: mierr
+      type(PlotParams) :: params
+      real(kind=8), allocatable :: vals(:, :, :)
+      character(len=20) :: filename
+      integer(c_int) :: rc
       if (.not. ads_sync_every_step) call SyncSolution(ads, ads_data, mierr)
+      params = PlotParams(0.d0, 1.d0, 0.d0, 1.d0, 0.d0, 1.d0, 31, 31, 31)
+      allocate(vals(params%resx, params%resy, params%resz))
+      rc = ads_eval_grid(ads%handle, [params%resx, params%resy, params%resz], &
+                         [params%startx, params%starty, params%startz], [params%endx, params%endy, params%endz], vals)
+      if (rc /= 0) then
+         write(*, '(a)') 'PrintSolution: '//ads_error_text(ads%handle)
+         return
+      end if
+      write(filename, '(i10)') iter
+      filename = 'step'//adjustl(filename)
+      call VtkOutput(trim(filename), vals, params)
    end subroutine PrintSolution
 
    ! ADSS::Cleanup (src/ADS.F90:802-863)

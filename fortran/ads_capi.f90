@@ -75,6 +75,13 @@ module ads_capi
          integer(c_int), value :: prec, refine
          real(c_double), intent(out) :: l2norm
       end function
+      integer(c_int) function ads_eval_grid(ctx, res, lo, hi, vals) bind(C, name="ads_eval_grid")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(in) :: res(3)
+         real(c_double), intent(in) :: lo(3), hi(3)
+         real(c_double), intent(out) :: vals(*)
+      end function
    end interface
 
 contains

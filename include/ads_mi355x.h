@@ -39,8 +39,8 @@ extern "C" {
 
 /* precision forms */
 #define ADS_PREC_FP64 0    /* FP64 assembly and sweeps                            */
-#define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 Kronecker residual refinement (constant kappa) */
-#define ADS_PREC_FP32_IR_KRON 2 /* FP32 sweeps on FP64 Kronecker residuals from a zero start (constant kappa) */
+#define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 residual refinement */
+#define ADS_PREC_FP32_IR_KRON 2 /* FP32 sweeps on FP64 residuals from a zero start */
 
 /* kernels for ads_time_kernel */
 #define ADS_KERNEL_HEAT_FP64 0      /* heat RHS assembly, FP64 */
@@ -106,10 +106,16 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  * integrand (problems/heat/RHS_eq.F90:69-81, f = 0, kq = kappa_e):
  *   F_a = sum_e sum_q J W (v_a u - dt kappa_e grad u . grad v_a)
  *   c  <- (Mz (x) My (x) Mx)^-1 F      (Sub_Step, ADS.F90:471-794)
- * prec = ADS_PREC_FP64, ADS_PREC_FP32_IR or ADS_PREC_FP32_IR_KRON (refine =
- * number of FP64-residual refinement passes, 1..4).  l2norm (may be NULL: no
- * host sync) receives sum J W u_n^2 of the state before the step
- * (RHS_eq.F90:81); the Kronecker form computes it only when l2norm != NULL. */
+ * prec = ADS_PREC_FP64: FP64 throughout (refine ignored).
+ * prec = ADS_PREC_FP32_IR: FP32 assembly and sweeps, then refine = 1..4
+ *   corrections, each an FP64 residual and FP32 sweeps.
+ * prec = ADS_PREC_FP32_IR_KRON: refine = 1..4 FP32 sweeps in all, on FP64
+ *   residuals from a zero start (the first residual is the right-hand side).
+ * The FP64 residual is the Kronecker operator A c - M x for a constant
+ * diffusivity, and b - M x with b = A(kappa) c from the FP64 element
+ * assembly for a per-element one.  l2norm (may be NULL: no host sync)
+ * receives sum J W u_n^2 of the state before the step (RHS_eq.F90:81),
+ * always computed in FP64. */
 int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm);
 
 /* Form3DRHS after FormUn (projection_engine.F90:154-284) on the context's
@@ -138,9 +144,24 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
  * gradients).  Given c_n and lam_next = dL/dc_{n+1}:
  *   mu = M^-T lam_next,  lam_out = A(kappa) mu = dL/dc_n,
  *   dkappa[e] += -dt sum_{q in e} J W grad c_n . grad mu   (if dkappa_d != NULL).
- * lam_out may equal lam_next, not c_n. */
+ * prec = ADS_PREC_FP64: all in FP64 (refine ignored).  ADS_PREC_FP32_IR or
+ * ADS_PREC_FP32_IR_KRON: mu by FP32 transposed sweeps with refine-1 FP64
+ * mass-residual corrections (refine = 1..4), then the assembly applied to
+ * mu and the diffusivity gradient in FP32 arithmetic (dkappa accumulated in
+ * FP64).  lam_out may equal lam_next, not c_n. */
 int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next_d,
-                       const double* kappa_d, double dt, double* lam_out_d, double* dkappa_d);
+                       const double* kappa_d, double dt, int prec, int refine, double* lam_out_d,
+                       double* dkappa_d);
+
+/* ---- output ---------------------------------------------------------------- */
+
+/* The spline with the context's coefficients on a tensor grid, for
+ * PrintSolution (src/ADS.F90:955-989): SaveSplinePlot (src/plot.F90:97-145)
+ * samples x_i = lerp(i/(res_x-1), lo_x, hi_x) (math.F90:44-51), likewise y
+ * and z, and evaluates EvalSpline with d = 0 (src/basis.F90:323-369) at each
+ * point.  vals (host, res_x*res_y*res_z doubles) is x fastest, the order
+ * VtkOutput writes (src/vtk.F90:55-60).  res = 2..65536 per direction. */
+int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const double hi[3], double* vals);
 
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 
@@ -149,6 +170,11 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
  * (2*(p+1)*(p+1)*nelem) in the layouts of ads_get_knots/ads_get_basis.
  * nelem = n - p + 1.  Any pointer may be NULL. */
 int ads_host_basis(int n, int p, double* U, int* O, double* J, double* W, double* X, double* NN);
+
+/* The per-direction tables ads_eval_grid uses, without a device: for the res
+ * points u_i = lerp(i/(res-1), lo, hi), first[i] = FindSpan - p
+ * (basis.F90:226-259) and vals[i*(p+1)+a] = the p+1 B-splines at u_i. */
+int ads_host_grid_tables(int n, int p, int res, double lo, double hi, int* first, double* vals);
 
 /* ---- synthetic inputs (host) --------------------------------------------- */
 

@@ -1,5 +1,7 @@
-"""The Fortran drop-in (fortran/: Setup, Interfaces, ADSS, projection_engine
-over the C ABI) driving the heat problem, against the oracle."""
+"""The Fortran drop-in (fortran/: Setup, Interfaces, ADSS, projection_engine,
+plot, vtk over the C ABI) driving the heat problem, against the oracle: the
+printed l2norms, the final coefficients and the step<iter>.vti files that
+PrintSolution writes (the spline on the 31^3 grid, src/ADS.F90:955-989)."""
 import os
 import subprocess
 
@@ -21,16 +23,16 @@ def driver():
     return DRIVER
 
 
-def test_driver_fails_loudly_without_gpu(driver):
+def test_driver_fails_loudly_without_gpu(driver, tmp_path):
     if _has_gpu():
         pytest.skip("a GPU is present")
-    res = subprocess.run([driver, "17", "2", "2", "1e-5", "1"], capture_output=True, text=True)
+    res = subprocess.run([driver, "17", "2", "2", "1e-5", "1"], capture_output=True, text=True, cwd=tmp_path)
     assert res.returncode != 0
     assert "ADS error" in res.stdout
 
 
-def test_driver_usage_error(driver):
-    res = subprocess.run([driver], capture_output=True, text=True)
+def test_driver_usage_error(driver, tmp_path):
+    res = subprocess.run([driver], capture_output=True, text=True, cwd=tmp_path)
     assert res.returncode != 0
     assert "usage" in res.stdout
 
@@ -41,18 +43,40 @@ def test_fortran_heat_run_matches_oracle(driver, orc, tmp_path):
     5 explicit steps): per-step l2norm and final coefficients vs the oracle."""
     out = tmp_path / "c.bin"
     res = subprocess.run([driver, "17", "2", "5", "1e-5", "1", str(out)], capture_output=True, text=True,
-                         timeout=120)
+                         timeout=120, cwd=tmp_path)
     print(res.stdout, res.stderr, flush=True)
     assert res.returncode == 0
     l2 = [float(line.split()[-1]) for line in res.stdout.splitlines() if "L2 norm" in line]
     S = orc.Setup(17, 2)
     c, l2o = S.project(orc.F0_SINPROD, orc.MODE_FAITHFUL)
     ref = [l2o]
+    states = [c]
     for _ in range(5):
         c, l2s = S.step(c, 1e-5, mode=orc.MODE_FAITHFUL)
         ref.append(l2s)
+        states.append(c)
     assert len(l2) == 6
     for a, b in zip(l2, ref):
         assert abs(a - b) <= 1e-12 * abs(b)
     got = np.fromfile(out, dtype=np.float64)
     assert np.linalg.norm(got - c) <= 1e-12 * np.linalg.norm(c)
+
+    # PrintSolution's files: step0.vti .. step5.vti, each the oracle spline of
+    # that step's coefficients on the 31^3 grid, printed with 10 decimals
+    U = orc.knots(17, 2)
+    t = np.arange(31) / 30.0
+    for it in (0, 3, 5):
+        lines = (tmp_path / ("step%d.vti" % it)).read_text().splitlines()
+        assert lines[:6] == ['<?xml version="1.0"?>', '<VTKFile type="ImageData" version="0.1">',
+                             '  <ImageData WholeExtent="0    30 0    30 0    30" origin="0 0 0" spacing="1 1 1">',
+                             '    <Piece Extent="0    30 0    30 0    30">', '      <PointData Scalars="Result">',
+                             '        <DataArray Name="Result" type="Float32">']
+        assert lines[-5:] == ['        </DataArray>', '      </PointData>', '    </Piece>', '  </ImageData>',
+                              '</VTKFile>']
+        vals = np.array([float(v) for v in lines[6:-5]])
+        assert vals.size == 31 ** 3 and all(v.startswith("          ") for v in lines[6:8])
+        cf = np.ascontiguousarray(states[it])
+        exp = np.array([orc.lib().orc_eval_spline(0, orc._d(U), 2, 17, orc._d(U), 2, 17, orc._d(U), 2, 17,
+                                                  orc._d(cf), x, y, z)
+                        for z in t for y in t for x in t])
+        assert np.max(np.abs(vals - exp)) <= 6e-11

@@ -20,7 +20,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-from ads_mpi_amd import adss  # noqa: E402
+from ads_mpi_amd import _lib, adss  # noqa: E402
 from ads_mpi_amd.autograd import HeatSolver, heat_rollout  # noqa: E402
 
 
@@ -109,5 +109,43 @@ def test_gradient_vs_finite_differences_config4():
             worst = max(worst, err)
         say("FD worst relative error", worst, "rollout rounding floor (L2 of a repeat difference)", noise)
         assert worst <= 1e-5
+    finally:
+        solver.close()
+
+
+def test_config5_fp32_forms_gradient():
+    """384x384x256 elements, p = 3, lognormal(0, 0.5) diffusivity from seed 5,
+    40 steps, checkpoint 8, dt = 5e-9 (stable for the diffusivity's range):
+    the forward in FP32 + refinement and the adjoint in FP32 with FP64
+    mass-residual corrections, against the all-FP64 rollout and gradient."""
+    n, p, dt, steps, ck = (386, 386, 258), 3, 5e-9, 40, 8
+    say("case config5")
+    solver = HeatSolver(n, p)
+    try:
+        D, E = solver.D, solver.E
+        c0 = torch.from_numpy(adss.uniform(5, D, -1.0, 1.0)).cuda()
+        kap = torch.from_numpy(adss.lognormal(5, E, 0.0, 0.5)).cuda()
+        c_obs = torch.from_numpy(adss.uniform(6, D, -1.0, 1.0)).cuda()
+        res = {}
+        for name, kw in (("fp64", dict(prec=_lib.ADS_PREC_FP64)),
+                         ("fp32", dict(prec=_lib.ADS_PREC_FP32_IR_KRON, refine=2,
+                                       adjoint_prec=_lib.ADS_PREC_FP32_IR_KRON, adjoint_refine=2))):
+            cg = c0.clone().requires_grad_(True)
+            kg = kap.clone().requires_grad_(True)
+            cT = heat_rollout(cg, kg, solver, dt, steps, ck, **kw)
+            loss = 0.5 * ((cT - c_obs) ** 2).sum()
+            loss.backward()
+            res[name] = (cT.detach(), cg.grad, kg.grad)
+            torch.cuda.synchronize()
+        e_state = float(torch.linalg.norm(res["fp32"][0] - res["fp64"][0]) / torch.linalg.norm(res["fp64"][0]))
+        e_gc = float(torch.linalg.norm(res["fp32"][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))
+        e_gk = float(torch.linalg.norm(res["fp32"][2] - res["fp64"][2]) / torch.linalg.norm(res["fp64"][2]))
+        say("config5 rel err: state", e_state, "grad c0", e_gc, "grad kappa", e_gk)
+        # each refined step is within 1e-10 of the FP64 step and the step is a
+        # contraction, so 40 steps stay within 40 x 1e-10
+        assert e_state <= 4e-9
+        # FP32 arithmetic in the adjoint assembly: ~1e-6 per step
+        assert e_gc <= 5e-5
+        assert e_gk <= 5e-4
     finally:
         solver.close()

@@ -223,7 +223,8 @@ def test_adjoint_vs_oracle(orc):
         out = torch.empty_like(ld)
         dk = torch.zeros(S.E, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        _lib.check(C.L.ads_step_adjoint_d(C.h, vp(cd), vp(ld), vp(kd), dt, vp(out), vp(dk)), C.h)
+        _lib.check(C.L.ads_step_adjoint_d(C.h, vp(cd), vp(ld), vp(kd), dt, _lib.ADS_PREC_FP64, 0, vp(out), vp(dk)),
+                   C.h)
         _lib.check(C.L.ads_synchronize(C.h), C.h)
         assert rel(out.cpu().numpy(), lam_o) <= TOL
         assert rel(dk.cpu().numpy(), dk_o) <= 1e-11
@@ -271,3 +272,71 @@ def test_kron_ir_l2norm_and_state_api():
         assert abs(l2a - l2b) <= 1e-12 * abs(l2a)
     finally:
         adss.Cleanup(ads, data)
+
+
+def test_refinement_forms_with_per_element_kappa():
+    """The refinement forms with a per-element diffusivity (the residual is
+    b - M x with b from the FP64 element assembly): <= 1e-10 relative against
+    the FP64 step with the same kappa, also for an in-place step."""
+    n, p, dt = (41, 38, 36), 3, 1e-7
+    C = Ctx(n, p)
+    try:
+        say("case refinement forms, per-element kappa")
+        rng = np.random.default_rng(8)
+        c = rng.uniform(-1, 1, C.D)
+        E = int(np.prod(C.ads.nelem))
+        kappa = adss.lognormal(5, E, 0.0, 0.5)
+        ref = C.step_d(c, dt, kappa=kappa, prec=0)
+        for prec, refine in ((1, 1), (2, 2), (2, 3)):
+            got = C.step_d(c, dt, kappa=kappa, prec=prec, refine=refine)
+            err = rel(got, ref)
+            say("prec", prec, "refine", refine, "rel err", err)
+            assert err <= 1e-10
+        # in place
+        cd, kd = dev(c), dev(kappa)
+        torch.cuda.synchronize()
+        _lib.check(C.L.ads_step_d(C.h, vp(cd), vp(cd), vp(kd), dt, 2, 2, None), C.h)
+        _lib.check(C.L.ads_synchronize(C.h), C.h)
+        assert rel(cd.cpu().numpy(), ref) <= 1e-10
+    finally:
+        C.close()
+
+
+def test_fp32_adjoint_against_fp64_adjoint(orc):
+    """The FP32 adjoint (FP32 transposed sweeps with FP64 mass-residual
+    corrections, FP32 assembly of mu and of the diffusivity gradient) against
+    the FP64 adjoint: lam and dkappa to 1e-5 relative L2, and the FP64 adjoint
+    itself against the oracle.  mu = M^-T lam of a random lam is rough, so
+    forming A(kappa) mu in FP32 cancels: ~50 FP32 unit roundoffs (2.8e-6
+    measured) rather than a few."""
+    n, p, dt = (33, 30, 28), 3, 1e-7
+    C = Ctx(n, p)
+    S = orc.Setup(n, p)
+    try:
+        say("case fp32 adjoint")
+        rng = np.random.default_rng(9)
+        c = rng.uniform(-1, 1, C.D)
+        lam = rng.uniform(-1, 1, C.D)
+        kappa = adss.lognormal(5, S.E, 0.0, 0.5)
+        cd, ld, kd = dev(c), dev(lam), dev(kappa)
+        out = {}
+        for prec, refine in ((0, 0), (2, 2), (1, 3)):
+            o = torch.empty_like(ld)
+            dk = torch.zeros(S.E, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            _lib.check(C.L.ads_step_adjoint_d(C.h, vp(cd), vp(ld), vp(kd), dt, prec, refine, vp(o), vp(dk)), C.h)
+            _lib.check(C.L.ads_synchronize(C.h), C.h)
+            out[(prec, refine)] = (o.cpu().numpy(), dk.cpu().numpy())
+        dk_o = np.zeros(S.E)
+        lam_o = S.step_adjoint(c, lam, dt, kappa=kappa, dkappa=dk_o)
+        l64, k64 = out[(0, 0)]
+        assert rel(l64, lam_o) <= TOL
+        assert rel(k64, dk_o) <= 1e-11
+        for key in ((2, 2), (1, 3)):
+            l32, k32 = out[key]
+            el, ek = rel(l32, l64), rel(k32, k64)
+            say("adjoint form", key, "lam rel err", el, "dkappa rel err", ek)
+            assert el <= 1e-5
+            assert ek <= 1e-5
+    finally:
+        C.close()
