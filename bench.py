@@ -97,8 +97,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: no CPU baseline, no gradient, plus one device copy of the "
-                         "coefficient vector (known bytes) to calibrate FETCH_SIZE/WRITE_SIZE")
+                    help="short run for rocprofv3: no CPU baseline, no gradient, plus one cold 1 GiB device "
+                         "copy (known bytes) to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.profile:
         args.no_cpu_baseline = True
@@ -188,10 +188,16 @@ def main():
     _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_KRON_RESIDUAL, dt, args.steps, ctypes.byref(ms_k)), h)
     ms_res = ms_k.value
     if args.profile:
-        # calibration dispatch for the PMC byte counters: D doubles read, D written
-        F = torch.empty_like(cdev)
-        F.copy_(cdev)
+        # calibration dispatch for the PMC byte counters: a copy of 1 GiB whose
+        # source was evicted from the 256 MB MALL by 1 GiB written after it
+        n_cal = 1 << 27
+        a_cal = torch.ones(n_cal, dtype=torch.float64, device="cuda")
+        b_cal = torch.ones(n_cal, dtype=torch.float64, device="cuda")
+        c_cal = torch.empty_like(a_cal)
         torch.cuda.synchronize()
+        c_cal.copy_(a_cal)
+        torch.cuda.synchronize()
+        del a_cal, b_cal, c_cal
     flops = heat_kernel_flops(E, P)
     achieved = flops / (ms_asm * 1e-3) / 1e12
     traffic = None

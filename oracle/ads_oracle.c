@@ -589,6 +589,7 @@ typedef struct {
     double dt;
     const double* c;
     const double* kappa;
+    const double* cg;  /* coefficients of the gradient (MultiStep's dUn); NULL: c */
 } orc_form;
 
 /* Faithful element kernel: FormUn's value and gradient at every quadrature
@@ -617,14 +618,15 @@ static double element_faithful(const orc_setup* s, const orc_form* fm, int ex, i
                             for (int bz = 0; bz <= pz; ++bz) {
                                 size_t ind = (size_t)(ox + bx) + (size_t)sx * ((oy + by) + (size_t)sy * (oz + bz));
                                 double Uc = fm->c[ind];
+                                double Ug = fm->cg ? fm->cg[ind] : Uc;
                                 double v = NNA(s, 0, 0, bx, kx, ex) * NNA(s, 1, 0, by, ky, ey) * NNA(s, 2, 0, bz, kz, ez);
                                 double dvx = NNA(s, 0, 1, bx, kx, ex) * NNA(s, 1, 0, by, ky, ey) * NNA(s, 2, 0, bz, kz, ez);
                                 double dvy = NNA(s, 0, 0, bx, kx, ex) * NNA(s, 1, 1, by, ky, ey) * NNA(s, 2, 0, bz, kz, ez);
                                 double dvz = NNA(s, 0, 0, bx, kx, ex) * NNA(s, 1, 0, by, ky, ey) * NNA(s, 2, 1, bz, kz, ez);
                                 Uval = Uval + Uc * v;
-                                dux = dux + Uc * dvx;
-                                duy = duy + Uc * dvy;
-                                duz = duz + Uc * dvz;
+                                dux = dux + Ug * dvx;
+                                duy = duy + Ug * dvy;
+                                duz = duz + Ug * dvz;
                             }
                     int kk = kx + qx * (ky + qy * kz);
                     un[kk] = Uval; dun[0][kk] = dux; dun[1][kk] = duy; dun[2][kk] = duz;
@@ -847,13 +849,13 @@ static int form_rhs(const orc_setup* s, int mode, const orc_form* fm, double* F,
 int orc_form_rhs_heat(const orc_setup* s, int mode, const double* c, const double* kappa,
                       double dt, double* F, double* l2norm, int ez_begin, int ez_end)
 {
-    orc_form fm = {0, 0, dt, c, kappa};
+    orc_form fm = {0, 0, dt, c, kappa, NULL};
     return form_rhs(s, mode, &fm, F, l2norm, ez_begin, ez_end);
 }
 
 int orc_form_rhs_project(const orc_setup* s, int mode, int f0_kind, double* F, double* l2norm)
 {
-    orc_form fm = {1, f0_kind, 0.0, NULL, NULL};
+    orc_form fm = {1, f0_kind, 0.0, NULL, NULL, NULL};
     return form_rhs(s, mode, &fm, F, l2norm, 0, -1);
 }
 
@@ -865,8 +867,8 @@ int orc_form_rhs_project(const orc_setup* s, int mode, int f0_kind, double* F, d
  * faithful mode the 1-D matrix is rebuilt and refactored (ComputeMatrix +
  * SolveOneDirection, ADS.F90:553-561); otherwise the factors from setup are
  * reused.  Columns are solved one at a time as MUMPS job 3 does (:363-368). */
-static int solve_direction(const orc_setup* s, int d, int faithful, double* rhs, int ncol,
-                           char trans, int line_limit)
+static int solve_direction_mix(const orc_setup* s, int d, int faithful, const double* mixp, double* rhs,
+                               int ncol, char trans, int line_limit)
 {
     int n = s->dofs[d], p = s->p[d];
     int ldab = 3 * p + 1;
@@ -877,8 +879,8 @@ static int solve_direction(const orc_setup* s, int d, int faithful, double* rhs,
     if (faithful) {
         ab_own = (double*)malloc(sizeof(double) * ldab * n);
         ipiv_own = (int*)malloc(sizeof(int) * n);
-        const double mix[4] = {1.0, 0.0, 0.0, 0.0};
-        orc_mkbbt_band(s->U[d], p, s->n[d], s->nelem[d], mix, ab_own, ldab);
+        const double mass[4] = {1.0, 0.0, 0.0, 0.0};
+        orc_mkbbt_band(s->U[d], p, s->n[d], s->nelem[d], mixp ? mixp : mass, ab_own, ldab);
         if (orc_dgbtrf(n, p, p, ab_own, ldab, ipiv_own) != 0) {
             free(ab_own); free(ipiv_own);
             return -1;
@@ -893,6 +895,12 @@ static int solve_direction(const orc_setup* s, int d, int faithful, double* rhs,
     free(ab_own);
     free(ipiv_own);
     return 0;
+}
+
+static int solve_direction(const orc_setup* s, int d, int faithful, double* rhs, int ncol, char trans,
+                           int line_limit)
+{
+    return solve_direction_mix(s, d, faithful, NULL, rhs, ncol, trans, line_limit);
 }
 
 int orc_solve_kron(const orc_setup* s, int faithful, double* F, int line_limit)
@@ -936,6 +944,48 @@ int orc_heat_step(const orc_setup* s, int mode, const double* c_in, const double
     int rc = orc_form_rhs_heat(s, mode, c_in, kappa, dt, c_out, l2norm, 0, -1);
     if (rc) return rc;
     return orc_solve_kron(s, mode == ORC_MODE_FAITHFUL, c_out, 0);
+}
+
+/* Sub_Step's three solves with A = mix1 M + mix2 K + mix3 B + mix4 B^T in
+ * every direction (ComputeMatrix(..., mix, ...), ADS.F90:553-561, 632, 722),
+ * matrices rebuilt and factored as the reference does. */
+int orc_solve_kron_mix(const orc_setup* s, const double mix[4], double* F)
+{
+    int sz3[3] = {s->dofs[0], s->dofs[1], s->dofs[2]};
+    size_t D = (size_t)sz3[0] * sz3[1] * sz3[2];
+    double* F2 = (double*)malloc(sizeof(double) * D);
+    int rc = 0;
+    rc |= solve_direction_mix(s, 0, 1, mix, F, sz3[1] * sz3[2], 'N', 0);
+    orc_reorder_for_y(sz3, F, F2);
+    rc |= solve_direction_mix(s, 1, 1, mix, F2, sz3[0] * sz3[2], 'N', 0);
+    orc_reorder_for_z(sz3, F2, F);
+    rc |= solve_direction_mix(s, 2, 1, mix, F, sz3[0] * sz3[1], 'N', 0);
+    orc_reorder_for_x(sz3, F, F2);
+    memcpy(F, F2, sizeof(double) * D);
+    free(F2);
+    return rc;
+}
+
+/* MultiStep (ADS.F90:391-429) with the heat integrand: substep s = 1..3
+ * forms the right-hand side from the point values of c_in (Un, kept from
+ * substep 1's FormUn) and the gradients of the latest coefficients (dUn of
+ * FormUn(s)), then solves with mix(:, s) in x, y and z.  l2norm is the last
+ * substep's (all three equal sum J W u_n^2).  mix is column-major 4 x 3. */
+int orc_multistep(const orc_setup* s, const double mix[12], const double* c_in, const double* kappa, double dt,
+                  double* c_out, double* l2norm)
+{
+    size_t D = (size_t)s->dofs[0] * s->dofs[1] * s->dofs[2];
+    double* prev = (double*)malloc(sizeof(double) * D);
+    memcpy(prev, c_in, sizeof(double) * D);
+    int rc = 0;
+    for (int sub = 0; sub < 3 && !rc; ++sub) {
+        orc_form fm = {0, 0, dt, c_in, kappa, prev};
+        rc = form_rhs(s, ORC_MODE_FAITHFUL, &fm, c_out, l2norm, 0, -1);
+        if (!rc) rc = orc_solve_kron_mix(s, mix + 4 * sub, c_out);
+        if (!rc) memcpy(prev, c_out, sizeof(double) * D);
+    }
+    free(prev);
+    return rc;
 }
 
 int orc_project(const orc_setup* s, int mode, int f0_kind, double* c_out, double* l2norm)

@@ -73,6 +73,11 @@ int orc_form_rhs_project(const orc_setup* s, int mode, int f0_kind, double* F, d
  * line_limit > 0 solves only the first line_limit lines of each direction
  * (bounded timing samples); 0 solves all. */
 int orc_solve_kron(const orc_setup* s, int faithful, double* F, int line_limit);
+/* the three solves with A = mix1 M + mix2 K + mix3 B + mix4 B^T per direction */
+int orc_solve_kron_mix(const orc_setup* s, const double mix[4], double* F);
+/* MultiStep with the heat integrand (mix column-major 4 x 3) */
+int orc_multistep(const orc_setup* s, const double mix[12], const double* c_in, const double* kappa, double dt,
+                  double* c_out, double* l2norm);
 /* transposed solve M^T x = F (adjoint of orc_solve_kron) */
 int orc_solve_kron_T(const orc_setup* s, double* F);
 /* one explicit heat step c_out = M^-1 (M - dt K(kappa)) c_in */

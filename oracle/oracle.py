@@ -89,6 +89,8 @@ def lib():
         L.orc_form_rhs_project.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_solve_kron.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int]
         L.orc_solve_kron_T.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_solve_kron_mix.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_multistep.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]
         L.orc_heat_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, _dp, _dp]
         L.orc_project.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_heat_step_adjoint.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
@@ -323,6 +325,27 @@ class Setup:
         if rc:
             raise RuntimeError("solve_T rc=%d" % rc)
         return x
+
+    def solve_mix(self, F, mix):
+        """The three solves with mix1 M + mix2 K + mix3 B + mix4 B^T per direction."""
+        x = np.array(F, dtype=np.float64, copy=True).ravel()
+        m = np.ascontiguousarray(mix, dtype=np.float64)
+        rc = lib().orc_solve_kron_mix(self.h, _d(m), _d(x))
+        if rc:
+            raise RuntimeError("solve_mix rc=%d" % rc)
+        return x
+
+    def multistep(self, c, mix, dt, kappa=None):
+        """MultiStep with the heat integrand; mix[s] is substep s's (mix1..mix4)."""
+        c = np.ascontiguousarray(c, dtype=np.float64).ravel()
+        m = np.ascontiguousarray(np.asarray(mix, dtype=np.float64).reshape(3, 4))
+        out = self._vec()
+        l2 = ctypes.c_double(0.0)
+        kp = _d(np.ascontiguousarray(kappa, dtype=np.float64)) if kappa is not None else None
+        rc = lib().orc_multistep(self.h, _d(m), _d(c), kp, dt, _d(out), ctypes.byref(l2))
+        if rc:
+            raise RuntimeError("multistep rc=%d" % rc)
+        return out, l2.value
 
     def step(self, c, dt, kappa=None, mode=MODE_SUMFACT):
         c = np.ascontiguousarray(c, dtype=np.float64).ravel()

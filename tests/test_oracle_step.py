@@ -134,3 +134,71 @@ def test_generators(orc):
     assert abs(z.mean()) < 5e-3 and abs(z.std() - 0.5) < 5e-3
     assert np.array_equal(orc.uniform(9, 10, 0, 1), orc.uniform(9, 10, 0, 1))
     assert not np.array_equal(orc.uniform(9, 10, 0, 1), orc.uniform(10, 10, 0, 1))
+
+
+def indep_mkbbt(n, p):
+    """M, K, B, B^T of one direction built independently of the oracle (scipy
+    BSpline at numpy Gauss points): B(i, j) = int N_i' N_j, the insertion
+    order of MKBBT (projection_engine.F90:123-128)."""
+    U = mlv.knots(n, p)
+    xg, wg = np.polynomial.legendre.leggauss(p + 1)
+    N = n + 1
+    from scipy.interpolate import BSpline
+    vals_d = []
+    for j in range(N):
+        cc = np.zeros(N)
+        cc[j] = 1.0
+        s = BSpline(U, cc, p, extrapolate=False)
+        vals_d.append((s, s.derivative()))
+    M = np.zeros((N, N)); K = np.zeros((N, N)); B = np.zeros((N, N))
+    for e in range(n - p + 1):
+        a, b = U[p + e], U[p + e + 1]
+        x = (xg + 1.0) * (b - a) / 2.0 + a
+        w = wg * (b - a) / 2.0
+        v = np.array([np.nan_to_num(s(x)) for s, _ in vals_d])
+        d = np.array([np.nan_to_num(t(x)) for _, t in vals_d])
+        M += (v * w) @ v.T
+        K += (d * w) @ d.T
+        B += (d * w) @ v.T
+    return M, K, B, B.T
+
+
+MIXES = [[1.0, 2e-3, 0.0, 0.0], [1.0, 0.0, 0.02, 0.0], [0.5, 1e-3, 0.0, 0.02]]
+
+
+def test_solve_mix_is_dense_kronecker_inverse(orc):
+    n, p = (7, 8, 6), 2
+    S = orc.Setup(n, p)
+    mk = [indep_mkbbt(nd, p) for nd in n]
+    F = np.random.default_rng(31).standard_normal(S.D)
+    for mix in MIXES:
+        A = [sum(m * X for m, X in zip(mix, mk[d])) for d in range(3)]
+        x = S.solve_mix(F, mix)
+        assert np.linalg.norm(kron3(A[2], A[1], A[0]) @ x - F) <= 1e-12 * np.linalg.norm(F)
+
+
+def test_multistep_against_dense(orc):
+    """MultiStep (ADS.F90:391-429) with the heat integrand: substep s solves
+    (A_s (x) A_s (x) A_s) c_s = M c_n - dt K c_{s-1}, where the mass part uses
+    the point values of c_n (Un) and the stiffness part the gradients of the
+    latest coefficients (dUn); l2norm = c_n^T M c_n."""
+    n, p, dt = (7, 8, 6), 2, 2e-3
+    S = orc.Setup(n, p)
+    mk = [indep_mkbbt(nd, p) for nd in n]
+    (Mx, Kx, _, _), (My, Ky, _, _), (Mz, Kz, _, _) = mk
+    M3 = kron3(Mz, My, Mx)
+    K3 = kron3(Mz, My, Kx) + kron3(Mz, Ky, Mx) + kron3(Kz, My, Mx)
+    c = np.random.default_rng(32).standard_normal(S.D)
+    out, l2 = S.multistep(c, MIXES, dt)
+    prev = c
+    for mix in MIXES:
+        A = [sum(m * X for m, X in zip(mix, mk[d])) for d in range(3)]
+        prev = np.linalg.solve(kron3(A[2], A[1], A[0]), M3 @ c - dt * (K3 @ prev))
+    assert np.linalg.norm(out - prev) <= 1e-11 * np.linalg.norm(prev)
+    assert abs(l2 - c @ (M3 @ c)) <= 1e-12 * abs(l2)
+    # mass matrices in every substep: the first substep is the heat step
+    first, _ = S.multistep(c, [[1.0, 0.0, 0.0, 0.0]] * 3, dt)
+    one, _ = S.step(c, dt, mode=orc.MODE_FAITHFUL)
+    two = S.solve(M3 @ c - dt * (K3 @ one))
+    three = S.solve(M3 @ c - dt * (K3 @ two))
+    assert np.linalg.norm(first - three) <= 1e-12 * np.linalg.norm(three)
