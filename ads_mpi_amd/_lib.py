@@ -84,6 +84,7 @@ def load(path: str | None = None):
     L.ads_project.argtypes = [_vp, ctypes.c_int, _dp]
     L.ads_step.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
     L.ads_last_l2norm.argtypes = [_vp, _dp]
+    L.ads_multistep.argtypes = [_vp, ctypes.c_double * 12, ctypes.c_double, _dp]
     L.ads_form_rhs.argtypes = [_vp, ctypes.c_double, _dp, _dp]
     L.ads_form_rhs_d.argtypes = [_vp, _vp, _vp, ctypes.c_double, _vp, _vp]
     L.ads_solve_d.argtypes = [_vp, _vp, ctypes.c_int]

@@ -232,9 +232,20 @@ def Step(iter, RHS_fun, ads, ads_data):
 
 
 def MultiStep(iter, mix, RHS_fun, ads, ads_data):
-    """ADSS::MultiStep (src/ADS.F90:391-429): the alternating-direction
-    substeps with per-direction operators are not part of this build."""
-    raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "MultiStep (mix-based ADS substeps) is not implemented")
+    """ADSS::MultiStep (src/ADS.F90:391-429) with the heat form: mix is the
+    Fortran mix(4,3) (mix[:, s] for substep s) or its 3 columns as rows;
+    substep s solves with mix(1,s)M + mix(2,s)K + mix(3,s)B + mix(4,s)B^T in
+    every direction (ads_multistep).  Returns l2norm."""
+    if not isinstance(RHS_fun, HeatRHS):
+        raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "MultiStep needs the heat form")
+    L = _lib.load()
+    m = np.asarray(mix, dtype=np.float64)
+    cols = m.T if m.shape == (4, 3) else m.reshape(3, 4)
+    arr = (ctypes.c_double * 12)(*cols.ravel())
+    _set_kappa(ads_data, RHS_fun.kappa)
+    l2 = ctypes.c_double(0.0)
+    check(L.ads_multistep(ads_data.handle, arr, float(RHS_fun.Dt), ctypes.byref(l2)), ads_data.handle)
+    return l2.value
 
 
 def Form3DRHS(ads, ads_data, RHS_fun):

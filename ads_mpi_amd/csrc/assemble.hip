@@ -28,8 +28,9 @@
 // points, so k_project keeps the point-wise quadrature, sum-factorised one
 // direction at a time.
 //
-// The adjoint instance of k_heat (GRAD) also accumulates the per-element
-// diffusivity gradient.
+// The adjoint instance of k_heat (kHeatGrad) also accumulates the per-element
+// diffusivity gradient; the MultiStep instance (kHeatDual) takes the mass
+// part from one coefficient field and the stiffness part from another.
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -107,10 +108,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 
 // Every table pointer is a separate __restrict__ argument so the compiler
 // may read the wave-uniform y and z element matrices with scalar loads.
-// GRAD (adjoint, FP64): c is mu = M^-T lam and, in the same pass,
+// MODE kHeatGrad (adjoint): c is mu = M^-T lam and, in the same pass,
 // dkappa[e] += -dt cn_e^T K_e mu_e (K_e symmetric) from a third LDS ring
 // holding the forward state cn.
-template <typename T, int P, bool GRAD>
+// MODE kHeatDual (MultiStep, src/ADS.F90:391-429): F = M c - dt kappa K cn,
+// the point values from c (Un) and the gradients from cn (dUn of the latest
+// substep); cn sits in the third ring.
+constexpr int kHeatPlain = 0, kHeatGrad = 1, kHeatDual = 2;
+
+template <typename T, int P, int MODE>
 __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restrict__ c, T* __restrict__ F,
                                                          const T* __restrict__ emx, const T* __restrict__ emy,
                                                          const T* __restrict__ emz, const double* __restrict__ kappa,
@@ -119,13 +125,14 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
 {
     constexpr int KL = P - 1, W = kWave + KL, R = kRowsWg + KL, NT = P * (P + 1) / 2;
     constexpr int NTH = kWave * kRowsWg;
+    constexpr bool GRAD = MODE == kHeatGrad, DUAL = MODE == kHeatDual, TWO = GRAD || DUAL;
     // The right-hand-side ring accumulates in FP64 also for T = float: on
     // gfx950 ds_add_f32 issues about three times slower than ds_add_f64
     // (LDS-active cycles 302M against 108M for the same instruction count).
     using A = double;
     __shared__ T cs[P][R][W];  // coefficient planes, ring by z
     __shared__ A fs[P][R][W];  // right-hand-side planes, ring by z
-    __shared__ T ns[GRAD ? P : 1][GRAD ? R : 1][GRAD ? W : 1];  // forward-state planes (GRAD)
+    __shared__ T ns[TWO ? P : 1][TWO ? R : 1][TWO ? W : 1];  // second field (cn)
     __shared__ double l2w[kRowsWg];
 
     const int tid = threadIdx.x;
@@ -157,7 +164,7 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
     // thread) and written to its ring slot later, so the loads of the next
     // plane overlap the current layer's work
     constexpr int NE = R * W, NPT = (NE + NTH - 1) / NTH;
-    T pc[NPT], pn[GRAD ? NPT : 1];
+    T pc[NPT], pn[TWO ? NPT : 1];
     auto fetch = [&](int z) {
         const long off = ex0 + static_cast<long>(sx) * ey0 + plane * z;
 #pragma unroll
@@ -166,7 +173,7 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
             const int r = i / W, col = i - r * W;
             const bool in = i < NE && col < ncol && r < nrow;
             pc[k] = in ? T(c[off + col + static_cast<long>(sx) * r]) : T(0);
-            if constexpr (GRAD) pn[k] = in ? T(cn[off + col + static_cast<long>(sx) * r]) : T(0);
+            if constexpr (TWO) pn[k] = in ? T(cn[off + col + static_cast<long>(sx) * r]) : T(0);
         }
     };
     auto stash = [&](int z) {
@@ -177,7 +184,7 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
             if (i < NE) {
                 const int r = i / W, col = i - r * W;
                 cs[slot][r][col] = pc[k];
-                if constexpr (GRAD) ns[slot][r][col] = pn[k];
+                if constexpr (TWO) ns[slot][r][col] = pn[k];
             }
         }
     };
@@ -227,50 +234,69 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
                     mr[b] = active ? Ex[(axo * 2) * P + b] : T(0);
                     kr[b] = active ? Ex[(axo * 2 + 1) * P + b] : T(0);
                 }
-                // x: mx = (Mx c)[axo], kx = (Kx c)[axo]   [ay][az]
-                T mx[P][P], kx[P][P];
+                // x: mx = (Mx c)[axo], kx = (Kx c)[axo]   [ay][az]; the
+                // stiffness source is cn in DUAL, with mk = (Mx cn)[axo]
+                T mx[P][P], kx[P][P], mk[DUAL ? P : 1][DUAL ? P : 1];
 #pragma unroll
                 for (int ay = 0; ay < P; ++ay)
 #pragma unroll
                     for (int az = 0; az < P; ++az) {
                         const T* col = &cs[slot[az]][wv + ay][lane];
+                        const T* colk = col;
+                        if constexpr (DUAL) colk = &ns[slot[az]][wv + ay][lane];
                         T a = mr[0] * col[0];
-                        T k = kr[0] * col[0];
+                        T k = kr[0] * colk[0];
 #pragma unroll
                         for (int b = 1; b < P; ++b) {
                             a = fma(mr[b], col[b], a);
-                            k = fma(kr[b], col[b], k);
+                            k = fma(kr[b], colk[b], k);
                         }
                         mx[ay][az] = a;
                         kx[ay][az] = k;
+                        if constexpr (DUAL) {
+                            T q = mr[0] * colk[0];
+#pragma unroll
+                            for (int b = 1; b < P; ++b) q = fma(mr[b], colk[b], q);
+                            mk[ay][az] = q;
+                        }
                     }
 #pragma unroll
                 for (int azo = 0; azo < P; ++azo) {
-                    // z: zm = Mz mx, zk = Mz kx + Kz mx   (output plane azo)   [ay]
-                    T zm[P], zk[P];
+                    // z: zm = Mz mx, zk = Mz kx + Kz mk, zq = Mz mk (DUAL; mk = mx otherwise)   [ay]
+                    T zm[P], zk[P], zq[DUAL ? P : 1];
 #pragma unroll
                     for (int ay = 0; ay < P; ++ay) {
+                        const T* mkr = &mx[ay][0];
+                        if constexpr (DUAL) mkr = &mk[ay][0];
                         T m = Ez[tri<P>(azo, 0)] * mx[ay][0];
-                        T k = fma(Ez[NT + tri<P>(azo, 0)], mx[ay][0], Ez[tri<P>(azo, 0)] * kx[ay][0]);
+                        T k = fma(Ez[NT + tri<P>(azo, 0)], mkr[0], Ez[tri<P>(azo, 0)] * kx[ay][0]);
 #pragma unroll
                         for (int az = 1; az < P; ++az) {
                             m = fma(Ez[tri<P>(azo, az)], mx[ay][az], m);
                             k = fma(Ez[tri<P>(azo, az)], kx[ay][az], k);
-                            k = fma(Ez[NT + tri<P>(azo, az)], mx[ay][az], k);
+                            k = fma(Ez[NT + tri<P>(azo, az)], mkr[az], k);
                         }
                         zm[ay] = m;
                         zk[ay] = k;
+                        if constexpr (DUAL) {
+                            T q = Ez[tri<P>(azo, 0)] * mkr[0];
+#pragma unroll
+                            for (int az = 1; az < P; ++az) q = fma(Ez[tri<P>(azo, az)], mkr[az], q);
+                            zq[ay] = q;
+                        }
                     }
-                    // y: M c = My zm, K c = My zk + Ky zm   (output row ayo)
+                    // y: M c = My zm, K c = My zk + Ky zq   (output row ayo)
 #pragma unroll
                     for (int ayo = 0; ayo < P; ++ayo) {
+                        const T* zqr = &zm[0];
+                        if constexpr (DUAL) zqr = &zq[0];
                         T om = Ey[tri<P>(ayo, 0)] * zm[0];
-                        T ok = fma(Ey[NT + tri<P>(ayo, 0)], zm[0], Ey[tri<P>(ayo, 0)] * zk[0]);
+                        T ok = fma(Ey[NT + tri<P>(ayo, 0)], zqr[0], Ey[tri<P>(ayo, 0)] * zk[0]);
 #pragma unroll
                         for (int ay = 1; ay < P; ++ay) {
                             om = fma(Ey[tri<P>(ayo, ay)], zm[ay], om);
                             ok = fma(Ey[tri<P>(ayo, ay)], zk[ay], ok);
-                            ok = fma(Ey[NT + tri<P>(ayo, ay)], zm[ay], ok);
+                            ok = fma(Ey[NT + tri<P>(ayo, ay)], zqr[ay], ok);
                         }
                         l2 = fma(cs[slot[azo]][wv + ayo][lane + axo], om, l2);
                         if constexpr (GRAD) gacc = fma(ns[slot[azo]][wv + ayo][lane + axo], ok, gacc);
@@ -455,14 +481,14 @@ AsmDims base_dims(const Tables<T>& tb)
     return dm;
 }
 
-template <typename T, int P, bool GRAD>
+template <typename T, int P, int MODE>
 hipError_t heat_p(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
                   int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
 {
     // two rounds of resident workgroups (a partly filled last round costs
     // as much as a full one)
     static const int resident =
-        resident_groups(reinterpret_cast<const void*>(&k_heat<T, P, GRAD>), kWave * kRowsWg, 0);
+        resident_groups(reinterpret_cast<const void*>(&k_heat<T, P, MODE>), kWave * kRowsWg, 0);
     AsmDims dm = base_dims(tb);
     dm.yblocks = (dm.ney + kRowsWg - 1) / kRowsWg;
     dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.yblocks, dm.nez, 2L * resident);
@@ -470,7 +496,7 @@ hipError_t heat_p(const double* c, T* F, const Tables<T>& tb, double dt, const d
     dim3 grid(dm.xchunks * dm.yblocks * nseg);
     *nparts = static_cast<int>(grid.x);
     if (*nparts > part_cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_heat<T, P, GRAD>), grid, dim3(kWave * kRowsWg), 0, s, c, F, tb.d[0].emat,
+    hipLaunchKernelGGL((k_heat<T, P, MODE>), grid, dim3(kWave * kRowsWg), 0, s, c, F, tb.d[0].emat,
                        tb.d[1].emat_sym, tb.d[2].emat_sym, kappa, static_cast<T>(-dt), dm, l2_part, cn, dkappa);
     return hipGetLastError();
 }
@@ -497,7 +523,7 @@ hipError_t assemble_p(int form, int f0, const double* c, T* F, const Tables<T>& 
                       double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
     return form == kFormHeat
-               ? heat_p<T, P, false>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, nullptr, nullptr, s)
+               ? heat_p<T, P, kHeatPlain>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, nullptr, nullptr, s)
                : project_p<T, P>(F, tb, f0, l2_part, part_cap, nparts, s);
 }
 
@@ -582,10 +608,22 @@ hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, T* la
     if (!dkappa)
         return launch_assemble<T>(p, kFormHeat, 0, mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, s);
     switch (p) {
-    case 1: return heat_p<T, 2, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 2: return heat_p<T, 3, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 3: return heat_p<T, 4, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 4: return heat_p<T, 5, true>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 1: return heat_p<T, 2, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 2: return heat_p<T, 3, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 3: return heat_p<T, 4, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 4: return heat_p<T, 5, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_heat_dual(int p, const double* c_val, const double* c_grad, double* F, const Tables<double>& tb,
+                            double dt, const double* kappa, double* l2_part, int part_cap, int* nparts, hipStream_t s)
+{
+    switch (p) {
+    case 1: return heat_p<double, 2, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 2: return heat_p<double, 3, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 3: return heat_p<double, 4, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 4: return heat_p<double, 5, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
     default: return hipErrorInvalidValue;
     }
 }

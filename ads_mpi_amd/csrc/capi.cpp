@@ -34,6 +34,13 @@ struct DevDir {
     T* bwd = nullptr;
 };
 
+// Device factor tables of one MultiStep operator mix(1)M + mix(2)K +
+// mix(3)B + mix(4)B^T (the same operator in x, y and z).
+struct MixSet {
+    double mix[4];
+    ads::SweepTables<double> sw[3];
+};
+
 }  // namespace
 
 struct ads_ctx {
@@ -67,6 +74,8 @@ struct ads_ctx {
     double* x64 = nullptr;
     double* b64 = nullptr;
     double* mu = nullptr;
+
+    std::vector<MixSet> mixsets;  // MultiStep operators factored so far
 
     std::vector<void*> allocs;
 };
@@ -699,6 +708,80 @@ int ads_fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double
     if (n < 0 || (n > 0 && !out)) return ADS_ERR_INVALID;
     ads::fill_lognormal(seed, n, mu, sigma, out);
     return ADS_OK;
+}
+
+namespace {
+
+// The factors of mix(1)M + mix(2)K + mix(3)B + mix(4)B^T per direction, built
+// on first use (MKBBT + band LU, as ComputeMatrix + SolveOneDirection do in
+// every substep) and kept for the context's lifetime.
+int mix_set(ads_ctx* ctx, const double mix[4], const ads::SweepTables<double>** out)
+{
+    for (const MixSet& m : ctx->mixsets)
+        if (m.mix[0] == mix[0] && m.mix[1] == mix[1] && m.mix[2] == mix[2] && m.mix[3] == mix[3]) {
+            *out = m.sw;
+            return ADS_OK;
+        }
+    if (ctx->mixsets.size() >= 64) return fail(ctx, ADS_ERR_INVALID, "MultiStep: more than 64 distinct mix columns");
+    MixSet m{};
+    for (int k = 0; k < 4; ++k) m.mix[k] = mix[k];
+    for (int d = 0; d < 3; ++d) {
+        const ads::Direction& dir = ctx->dir[d];
+        std::vector<double> band, lu, fwd, bwd;
+        std::vector<int> piv, fpiv;
+        ads::mix_band(dir, mix, band);
+        std::string e = ads::band_lu(dir.dofs, dir.p, band, lu, piv);
+        if (!e.empty()) return fail(ctx, ADS_ERR_SINGULAR, "MultiStep: direction " + std::to_string(d) + ": " + e);
+        ads::sweep_tables(dir.dofs, dir.p, lu, piv, fwd, fpiv, bwd);
+        double *dfwd = nullptr, *dbwd = nullptr;
+        int* dpiv = nullptr;
+        int rc;
+        if ((rc = upload(ctx, &dfwd, fwd))) return rc;
+        if ((rc = upload(ctx, &dpiv, fpiv))) return rc;
+        if ((rc = upload(ctx, &dbwd, bwd))) return rc;
+        m.sw[d].fwd = dfwd;
+        m.sw[d].piv = dpiv;
+        m.sw[d].bwd = dbwd;
+        m.sw[d].n = dir.dofs;
+        m.sw[d].pivots = 0;
+        for (int v : fpiv) m.sw[d].pivots |= (v != 0);
+    }
+    ctx->mixsets.push_back(m);
+    *out = ctx->mixsets.back().sw;
+    return ADS_OK;
+}
+
+}  // namespace
+
+int ads_multistep(ads_ctx* ctx, const double mix[12], double dt, double* l2norm)
+{
+    if (check_ctx(ctx) || !mix) return ADS_ERR_INVALID;
+    if (!std::isfinite(dt)) return fail(ctx, ADS_ERR_INVALID, "dt is not finite");
+    int rc;
+    if (!ctx->x64 && (rc = dev_alloc(ctx, &ctx->x64, ctx->D))) return rc;
+    // the sets are referenced by pointer below: no reallocation within the cap
+    ctx->mixsets.reserve(64);
+    const ads::SweepTables<double>* sw[3];
+    for (int sub = 0; sub < 3; ++sub)
+        if ((rc = mix_set(ctx, mix + 4 * sub, &sw[sub]))) return rc;
+    // substep outputs: F, x64, F (the last one is the new state)
+    double* out[3] = {ctx->F, ctx->x64, ctx->F};
+    const double* prev = ctx->c;
+    for (int sub = 0; sub < 3; ++sub) {
+        if (sub == 0) {
+            // values and gradients from the same (initial) coefficients
+            if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, ctx->c, out[0], dt, ctx->kappa, ctx->l2))) return rc;
+        } else {
+            if ((rc = zero(ctx, out[sub], ctx->D))) return rc;
+            int nparts = 0;
+            ADS_HIP(ctx, ads::launch_heat_dual(ctx->p, ctx->c, prev, out[sub], ctx->tb64, dt, ctx->kappa,
+                                               ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream));
+        }
+        ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, out[sub], sw[sub], false, ctx->stream));
+        prev = out[sub];
+    }
+    std::swap(ctx->c, ctx->F);
+    return read_l2(ctx, l2norm);
 }
 
 int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms_per_launch)

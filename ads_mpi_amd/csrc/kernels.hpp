@@ -65,6 +65,11 @@ hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, const int* f
                             const int* fy, const double* by, int ry, const int* fz, const double* bz, int rz,
                             double* out, hipStream_t s);
 
+// MultiStep's heat right-hand side: F = M c_val - dt kappa K c_grad (point
+// values from c_val, gradients from c_grad); l2 from c_val.  F zero on entry.
+hipError_t launch_heat_dual(int p, const double* c_val, const double* c_grad, double* F, const Tables<double>& tb,
+                            double dt, const double* kappa, double* l2_part, int part_cap, int* nparts, hipStream_t s);
+
 // Deterministic sum of n partials into out[0].
 hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStream_t s);
 

@@ -206,10 +206,11 @@ std::string build_direction(int n, int p, Direction& d)
                 }
             }
 
-    // 1-D mass and stiffness matrices, band form (half-bandwidth p)
+    // 1-D mass, stiffness and advection matrices, band form (half-bandwidth p)
     const int bw = 2 * p + 1;
     d.mass_band.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
     d.stiff_band.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
+    d.adv_band.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
     for (int e = 0; e < d.nelem; ++e)
         for (int iq = 0; iq < q; ++iq)
             for (int c = 0; c < P; ++c)
@@ -219,6 +220,7 @@ std::string build_direction(int n, int p, Direction& d)
                     const size_t at = static_cast<size_t>(col) * bw + (p + row - col);
                     d.mass_band[at] += t[c * 2] * t[dd * 2] * d.jac[e] * d.wts[iq];
                     d.stiff_band[at] += t[c * 2 + 1] * t[dd * 2 + 1] * d.jac[e] * d.wts[iq];
+                    d.adv_band[at] += t[c * 2 + 1] * t[dd * 2] * d.jac[e] * d.wts[iq];
                 }
 
     d.mass_rows.assign(static_cast<size_t>(d.dofs) * bw, 0.0);
@@ -231,14 +233,21 @@ std::string build_direction(int n, int p, Direction& d)
             d.stiff_rows[static_cast<size_t>(i) * bw + t] = d.stiff_band[static_cast<size_t>(j) * bw + (p + i - j)];
         }
 
-    // banded LU with partial pivoting (dgbtf2), kl = ku = p, ldab = 3p+1
-    const int kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1, N = d.dofs;
-    d.lu.assign(static_cast<size_t>(N) * ldab, 0.0);
-    d.piv.assign(N, 0);
-    auto AB = [&](int r, int j) -> double& { return d.lu[static_cast<size_t>(j) * ldab + r]; };
+    // banded LU of the mass matrix
+    return band_lu(d.dofs, p, d.mass_band, d.lu, d.piv);
+}
+
+std::string band_lu(int N, int p, const std::vector<double>& band, std::vector<double>& lu, std::vector<int>& piv)
+{
+    // partial pivoting (dgbtf2), kl = ku = p, ldab = 3p+1
+    const int bw = 2 * p + 1;
+    const int kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1;
+    lu.assign(static_cast<size_t>(N) * ldab, 0.0);
+    piv.assign(N, 0);
+    auto AB = [&](int r, int j) -> double& { return lu[static_cast<size_t>(j) * ldab + r]; };
     for (int j = 0; j < N; ++j)
         for (int i = std::max(0, j - ku); i <= std::min(N - 1, j + kl); ++i)
-            AB(kv + i - j, j) = d.mass_band[static_cast<size_t>(j) * bw + (p + i - j)];
+            AB(kv + i - j, j) = band[static_cast<size_t>(j) * bw + (p + i - j)];
     int reach = 0;  // last column touched by row interchanges so far
     for (int j = 0; j < N; ++j) {
         const int below = std::min(kl, N - 1 - j);
@@ -249,7 +258,7 @@ std::string build_direction(int n, int p, Direction& d)
                 bestv = std::fabs(AB(kv + t, j));
                 best = t;
             }
-        d.piv[j] = j + best;
+        piv[j] = j + best;
         if (AB(kv + best, j) == 0.0)
             return "singular 1-D matrix (zero pivot in column " + std::to_string(j + 1) + ")";
         reach = std::max(reach, std::min(j + ku + best, N - 1));
@@ -266,6 +275,19 @@ std::string build_direction(int n, int p, Direction& d)
         }
     }
     return std::string();
+}
+
+void mix_band(const Direction& d, const double mix[4], std::vector<double>& band)
+{
+    const int p = d.p, bw = 2 * p + 1, N = d.dofs;
+    band.assign(static_cast<size_t>(N) * bw, 0.0);
+    for (int j = 0; j < N; ++j)
+        for (int i = std::max(0, j - p); i <= std::min(N - 1, j + p); ++i) {
+            const size_t at = static_cast<size_t>(j) * bw + (p + i - j);
+            const size_t tr = static_cast<size_t>(i) * bw + (p + j - i);  // (j, i): B^T(i, j) = B(j, i)
+            band[at] = mix[0] * d.mass_band[at] + mix[1] * d.stiff_band[at] + mix[2] * d.adv_band[at] +
+                       mix[3] * d.adv_band[tr];
+        }
 }
 
 void dense_matrix(const Direction& d, const double mix[4], double* out)
@@ -290,18 +312,24 @@ void dense_matrix(const Direction& d, const double mix[4], double* out)
 void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>& fwd_piv,
                   std::vector<double>& bwd)
 {
-    const int N = d.dofs, p = d.p, kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1;
+    sweep_tables(d.dofs, d.p, d.lu, d.piv, fwd, fwd_piv, bwd);
+}
+
+void sweep_tables(int N, int p, const std::vector<double>& lu, const std::vector<int>& piv, std::vector<double>& fwd,
+                  std::vector<int>& fwd_piv, std::vector<double>& bwd)
+{
+    const int kl = p, ku = p, kv = kl + ku, ldab = 2 * kl + ku + 1;
     const int K = kl + ku;
     fwd.assign(static_cast<size_t>(N) * p, 0.0);
     fwd_piv.assign(N, 0);
     bwd.assign(static_cast<size_t>(N) * (K + 1), 0.0);
     for (int j = 0; j < N; ++j) {
-        fwd_piv[j] = d.piv[j] - j;
+        fwd_piv[j] = piv[j] - j;
         for (int t = 1; t <= p; ++t)
-            if (j + t < N) fwd[static_cast<size_t>(j) * p + t - 1] = d.lu[static_cast<size_t>(j) * ldab + kv + t];
-        bwd[static_cast<size_t>(j) * (K + 1)] = 1.0 / d.lu[static_cast<size_t>(j) * ldab + kv];
+            if (j + t < N) fwd[static_cast<size_t>(j) * p + t - 1] = lu[static_cast<size_t>(j) * ldab + kv + t];
+        bwd[static_cast<size_t>(j) * (K + 1)] = 1.0 / lu[static_cast<size_t>(j) * ldab + kv];
         for (int t = 1; t <= K; ++t)
-            if (j - t >= 0) bwd[static_cast<size_t>(j) * (K + 1) + t] = d.lu[static_cast<size_t>(j) * ldab + kv - t];
+            if (j - t >= 0) bwd[static_cast<size_t>(j) * (K + 1) + t] = lu[static_cast<size_t>(j) * ldab + kv - t];
     }
 }
 

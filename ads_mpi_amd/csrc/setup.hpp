@@ -35,6 +35,7 @@ struct Direction {
     // 1-D matrices in band form, row-major by column: band[j*(2p+1) + (p + i - j)] = A(i, j)
     std::vector<double> mass_band;
     std::vector<double> stiff_band;
+    std::vector<double> adv_band;  // B(i, j) = sum N_i' N_j J W (MKBBT's B, projection_engine.F90:125)
     // the same matrices by row: rows[i*(2p+1) + t] = A(i, i-p+t), zero outside
     std::vector<double> mass_rows;
     std::vector<double> stiff_rows;
@@ -48,6 +49,16 @@ struct Direction {
 // Builds the direction; returns an empty string or an error message.
 std::string build_direction(int n, int p, Direction& d);
 
+// Banded LU with partial pivoting (dgbtf2 semantics): band in the
+// column-major layout of mass_band (half-bandwidth p), result in the dgbtrf
+// layout of Direction::lu and 0-based pivot rows.  Returns an error message
+// for a zero pivot.
+std::string band_lu(int N, int p, const std::vector<double>& band, std::vector<double>& lu, std::vector<int>& piv);
+
+// mix(1)M + mix(2)K + mix(3)B + mix(4)B^T in the layout of mass_band
+// (ComputeMatrix -> MKBBT, projection_engine.F90:531-544, 63-132).
+void mix_band(const Direction& d, const double mix[4], std::vector<double>& band);
+
 // Dense (n+1)x(n+1) row-major mix(1)M + mix(2)K + mix(3)B + mix(4)B^T.
 void dense_matrix(const Direction& d, const double mix[4], double* out);
 
@@ -58,6 +69,9 @@ void dense_matrix(const Direction& d, const double mix[4], double* out);
 //   bwd[j*(2p+1) + t]   = U(j-t, j), t = 1..2p (0 before the start)
 void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>& fwd_piv,
                   std::vector<double>& bwd);
+// The same from any factors in the layout of Direction::lu / piv.
+void sweep_tables(int N, int p, const std::vector<double>& lu, const std::vector<int>& piv, std::vector<double>& fwd,
+                  std::vector<int>& fwd_piv, std::vector<double>& bwd);
 
 // FindSpan (A2.1, src/basis.F90:226-259): the knot span of u, with
 // u >= U(n+1) -> n and u <= U(p) -> p.

@@ -212,8 +212,10 @@ contains
       if (ads_sync_every_step) call SyncSolution(ads, ads_data, mierr)
    end subroutine Step
 
-   ! ADSS::MultiStep (src/ADS.F90:391-429): the mix-based alternating
-   ! substeps are not part of this build.
+   ! ADSS::MultiStep (src/ADS.F90:391-429): three substeps, substep s solving
+   ! with mix(1,s)M + mix(2,s)K + mix(3,s)B + mix(4,s)B^T in every direction;
+   ! the registered heat form supplies the integrand (values of the step's
+   ! initial state, gradients of the latest coefficients).
    subroutine MultiStep(iter, mix, RHS_fun, ads, ads_data, l2norm, mierr)
       integer(kind=4), intent(in) :: iter
       real(kind=8), intent(in) :: mix(4, 3)
@@ -222,9 +224,32 @@ contains
       type(ADS_compute_data), intent(inout) :: ads_data
       real(kind=8), intent(out) :: l2norm
       integer(kind=4), intent(out) :: mierr
+      type(c_funptr) :: fp
+      integer :: i
+      real(c_double) :: l2
+      integer(c_int) :: rc
+      fp = c_funloc(RHS_fun)
       l2norm = 0.d0
-      mierr = ADS_ERR_UNSUPPORTED
-      write(*, '(a)') 'MultiStep: mix-based alternating-direction substeps are not implemented'
+      do i = 1, nforms
+         if (c_associated(forms(i)%fptr, fp)) exit
+      end do
+      if (i > nforms) then
+         write(*, '(a)') 'MultiStep: RHS_fun is not bound to a device form (RegisterHeatForm)'
+         mierr = ADS_ERR_UNSUPPORTED
+         return
+      end if
+      if (forms(i)%kind /= FORM_HEAT) then
+         write(*, '(a)') 'MultiStep: only the heat form has substeps'
+         mierr = ADS_ERR_UNSUPPORTED
+         return
+      end if
+      rc = ads_multistep(ads%handle, mix, forms(i)%dt, l2)
+      call report(ads%handle, rc, 'MultiStep')
+      mierr = rc
+      if (rc /= ADS_OK) return
+      ads_data%t = ads_data%t + forms(i)%dt
+      l2norm = l2
+      if (ads_sync_every_step) call SyncSolution(ads, ads_data, mierr)
    end subroutine MultiStep
 
    ! ADSS::PrintSolution (src/ADS.F90:955-989): refreshes ads_data%F and

@@ -82,6 +82,13 @@ module ads_capi
          real(c_double), intent(in) :: lo(3), hi(3)
          real(c_double), intent(out) :: vals(*)
       end function
+      integer(c_int) function ads_multistep(ctx, mix, dt, l2norm) bind(C, name="ads_multistep")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: mix(4, 3)
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: l2norm
+      end function
    end interface
 
 contains

@@ -118,6 +118,19 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  * always computed in FP64. */
 int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm);
 
+/* ADSS::MultiStep(iter, mix, RHS_fun, ...) (src/ADS.F90:391-429) with the
+ * heat integrand, on the context's state: three substeps s = 1..3.  Substep
+ * s assembles (Form3DRHS, projection_engine.F90:154-284)
+ *   F_a = sum_e sum_q J W (v_a u_n - dt kappa_e grad u_{s-1} . grad v_a)
+ * with the point values of the step's initial state (Un of FormUn(1)) and
+ * the gradients of the latest coefficients (dUn of FormUn(s)), then solves
+ * with A_s (x) A_s (x) A_s, A_s = mix(1,s) M + mix(2,s) K + mix(3,s) B +
+ * mix(4,s) B^T in every direction (ComputeMatrix -> MKBBT,
+ * projection_engine.F90:531-544, 63-132; B(i,j) = sum N_i' N_j J W).  mix is
+ * column-major 4 x 3, Fortran's mix(4,3).  The factors of each distinct mix
+ * column are built once and kept.  l2norm (may be NULL) as ads_step. */
+int ads_multistep(ads_ctx* ctx, const double mix[12], double dt, double* l2norm);
+
 /* Form3DRHS after FormUn (projection_engine.F90:154-284) on the context's
  * coefficients and diffusivity, heat integrand at t > 0, without advancing:
  * F (host, x fastest) = A(kappa) c; l2norm (may be NULL) = sum J W u^2. */

@@ -340,3 +340,33 @@ def test_fp32_adjoint_against_fp64_adjoint(orc):
             assert ek <= 1e-5
     finally:
         C.close()
+
+
+@pytest.mark.parametrize("n,p", [((9, 10, 8), 2), ((11, 9, 10), 3), ((70, 12, 9), 2)])
+def test_multistep_vs_oracle(orc, n, p):
+    """ADSS::MultiStep (src/ADS.F90:391-429) with the heat form: three
+    substeps with mix(:, s) operators (mass, stiffness and advection terms,
+    non-symmetric ones included), against the oracle's faithful MultiStep."""
+    say("case multistep", n, p)
+    S = orc.Setup(n, p)
+    mixes = [[1.0, 2e-3, 0.0, 0.0], [1.0, 0.0, 0.02, 0.0], [0.5, 1e-3, 0.0, 0.02]]
+    dt = 1e-4
+    c = np.random.default_rng(41).uniform(-1, 1, S.D)
+    kappa = orc.lognormal(5, S.E, 0.0, 0.5)
+    ads, data = adss.initialize(n, p)
+    try:
+        for kap in (None, kappa):
+            data.F = c
+            rhs = adss.HeatRHS(Dt=dt, kappa=kap)
+            l2 = adss.MultiStep(1, np.array(mixes).T, rhs, ads, data)
+            ref, l2o = S.multistep(c, mixes, dt, kappa=kap)
+            say("kappa", kap is not None, "rel err", rel(data.F, ref))
+            assert rel(data.F, ref) <= TOL
+            assert abs(l2 - l2o) <= TOL * abs(l2o)
+        # mass operators in every substep: substep 1 is the heat step
+        data.F = c
+        adss.MultiStep(1, [[1.0, 0.0, 0.0, 0.0]] * 3, adss.HeatRHS(Dt=dt), ads, data)
+        ref, _ = S.multistep(c, [[1.0, 0.0, 0.0, 0.0]] * 3, dt)
+        assert rel(data.F, ref) <= TOL
+    finally:
+        adss.Cleanup(ads, data)
