@@ -7,7 +7,7 @@
 //   k_sweep_rows       the same along x, where a line is a contiguous row: a
 //                      wave stages 64 rows through LDS tiles (the on-device
 //                      replacement of Gather/ReorderRHSFor*/Scatter).
-//   k_kron_residual    FP64 Kronecker residual r = A c - M x for the
+//   k_kron_residual    FP64 Kronecker residual r = b + A c - M x for the
 //                      FP32 + refinement forms, one streaming pass.
 #include "kernels.hpp"
 
@@ -429,19 +429,32 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_sweep_rows(T* __restrict__
 // Kronecker residual r = b + A c - M x, A = M - dtk K (constant kappa), in
 // FP64 (c, x and b optional: nullptr means zero; with c = 0 and b = A(kappa)c
 // from the element assembly it is the mass residual b - M x of a variable
-// diffusivity):
-//   X: e = Mx(c - x) - dtk Kx c,   m = Mx c
-//   Y: f = My e - dtk Ky m,        g = My m
-//   Z: r = Mz f - dtk Kz g,        and M c = Mz g for l2 = c^T M c
-// in one pass.  A workgroup of four waves owns 64 x columns and four y rows
-// (one wave each) and streams a z segment: per input plane it stages the
-// c (and c - x) tile with its halos (64+2p by 4+2p) in LDS, applies the X
-// stage to all tile rows, the Y stage per wave, and keeps the last 2p+1 Y
-// results of its lane in registers for the Z stage.  HBM traffic is about one
-// read of c and x and one write of r per DOF.
+// diffusivity).  With d = c - x, one direction at a time:
+//   Y: yd = My d,   ym = My c,   yk = Ky c
+//   X: e = Mx yd - dtk (Kx ym + Mx yk),   g = Mx ym
+//   Z: r = Mz e - dtk Kz g   (+ b),  and M c = Mz g for l2 = c^T M c
+// (without x, d = c and e = g - dtk (Kx ym + Mx yk)).  A workgroup of eight
+// waves streams a z segment of a tile of eight y rows (one wave each) and 64
+// x columns, of which the middle 64 - 2p are its outputs:
+//   * each input plane's tile with its y halo (8+2p rows x 64) goes to a
+//     double-buffered LDS tile (one barrier per plane); the loads of the
+//     next plane are in flight meanwhile;
+//   * the Y stage runs on all 64 columns of the wave's row (the x halo costs
+//     2p idle lanes instead of recomputing halo rows), its results go to a
+//     row of LDS private to the wave, and the X stage reads its 2p+1
+//     neighbours there;
+//   * the Z stage keeps the last 2p+1 X results of the lane in registers.
+// About 56 (70 with x) multiply-adds per DOF; HBM traffic about one read of
+// c (and x, b) and one write of r per DOF.
 // Row tables hold A(i, i-KL+t), t = 0..2KL, zero outside the matrix.
 // ---------------------------------------------------------------------------
-constexpr int kResRows = 4;
+constexpr int kResRows = 8;
+
+// output columns per residual workgroup: the wave's width less the x halos
+__host__ __device__ constexpr int res_out_width(int p)
+{
+    return kWave - 2 * p;
+}
 
 struct ResDims {
     int nx, ny, nz;
@@ -457,20 +470,22 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return k * q + min(k, r) + s;
 }
 
-template <int P, typename TX, typename TR>
-__global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
+// four waves per SIMD (two workgroups per CU) up to p = 3; p = 4 keeps its
+// larger windows in registers at two
+template <int P, typename TX, typename TR, bool HX>
+__global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residual(
     const double* __restrict__ c, const TX* __restrict__ x, const double* __restrict__ b, TR* __restrict__ r,
-    const double* __restrict__ mx_rows,
-    const double* __restrict__ kx_rows, const double* __restrict__ my_rows, const double* __restrict__ ky_rows,
-    const double* __restrict__ mz_rows, const double* __restrict__ kz_rows, double dtk, ResDims dm,
-    double* __restrict__ l2_part)
+    const double* __restrict__ mx_rows, const double* __restrict__ kx_rows, const double* __restrict__ my_rows,
+    const double* __restrict__ ky_rows, const double* __restrict__ mz_rows, const double* __restrict__ kz_rows,
+    double dtk, ResDims dm, double* __restrict__ l2_part)
 {
     constexpr int KL = P - 1, BW = 2 * KL + 1;
-    constexpr int TYH = kResRows + 2 * KL, TXH = kWave + 2 * KL, NTH = kWave * kResRows;
-    __shared__ double tc[TYH][TXH];  // c tile, rows y0-KL.., columns x0-KL..
-    __shared__ double td[TYH][TXH];  // c - x tile
-    __shared__ double te[TYH][kWave];
-    __shared__ double tm[TYH][kWave];
+    constexpr int OW = res_out_width(KL);
+    constexpr int TYH = kResRows + 2 * KL, NTH = kWave * kResRows;
+    constexpr int NY = HX ? 3 : 2;  // Y results: ym, yk (, yd)
+    __shared__ double tc[2][TYH][kWave];                         // c tiles, rows y0-KL..
+    __shared__ double td[HX ? 2 : 1][HX ? TYH : 1][kWave];       // c - x tiles
+    __shared__ double sy[kResRows][NY][kWave + 2 * KL];          // Y results, column lane + KL
     __shared__ double l2w[kResRows];
 
     const int tid = threadIdx.x;
@@ -480,94 +495,99 @@ __global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
     const int xc = tile % dm.xchunks;
     const int yb = (tile / dm.xchunks) % dm.yblocks;
     const int seg = tile / (dm.xchunks * dm.yblocks);
-    const int x0 = xc * kWave, y0 = yb * kResRows;
+    const int x0 = xc * OW - KL, y0 = yb * kResRows;  // global column of lane 0, first output row
     const int gx = x0 + lane, gy = y0 + wv;
     const int zs = seg * dm.zseg;
     const int ze = min(zs + dm.zseg, dm.nz);
-    const bool xin = gx < dm.nx, yin = gy < dm.ny;
+    const bool xout = lane >= KL && lane < kWave - KL && gx < dm.nx;
+    const bool out = xout && gy < dm.ny;
     const long sxy = static_cast<long>(dm.nx) * dm.ny;
 
     double mxr[BW], kxr[BW];
+    const int gxc = min(max(gx, 0), dm.nx - 1);
 #pragma unroll
     for (int t = 0; t < BW; ++t) {
-        mxr[t] = xin ? mx_rows[static_cast<size_t>(gx) * BW + t] : 0.0;
-        kxr[t] = xin ? kx_rows[static_cast<size_t>(gx) * BW + t] : 0.0;
+        mxr[t] = mx_rows[static_cast<size_t>(gxc) * BW + t];
+        kxr[t] = kx_rows[static_cast<size_t>(gxc) * BW + t];
     }
     const int gyc = min(gy, dm.ny - 1);
     const double* __restrict__ myr = my_rows + static_cast<size_t>(gyc) * BW;
     const double* __restrict__ kyr = ky_rows + static_cast<size_t>(gyc) * BW;
 
-    double fw[BW], gw[BW], cw[KL + 1];  // z windows: f, g and the centre c
+    double fw[BW], gw[BW], cw[KL + 1];  // z windows: e, g and the centre c
 #pragma unroll
     for (int t = 0; t < BW; ++t) fw[t] = gw[t] = 0.0;
 #pragma unroll
     for (int t = 0; t <= KL; ++t) cw[t] = 0.0;
     double l2 = 0.0;
 
-    // the tiles of the next two input planes are fetched into registers
-    // while the current one is processed (NPT entries per thread each)
-    constexpr int NT = TYH * TXH, NPT = (NT + NTH - 1) / NTH;
+    // tiles of input plane zi (NPT entries per thread of c and x) and the b
+    // value of output plane zi - KL are fetched one plane ahead
+    constexpr int NT = TYH * kWave, NPT = (NT + NTH - 1) / NTH;
     const int zend = ze + KL;
-    auto fetch = [&](int zi, double* pc, double* px) {
+    auto fetch = [&](int zi, double* pc, TX* px, double& pb) {
         const bool zok = zi >= 0 && zi < dm.nz;
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
             const int i = tid + k * NTH;
-            const int rr = i / TXH, cc = i - rr * TXH;
-            const int yy = y0 - KL + rr, xx = x0 - KL + cc;
+            const int rr = i / kWave, cc = i - rr * kWave;
+            const int yy = y0 - KL + rr, xx = x0 + cc;
             const bool ok = i < NT && zok && yy >= 0 && yy < dm.ny && xx >= 0 && xx < dm.nx;
             const long at = xx + static_cast<long>(dm.nx) * yy + sxy * zi;
             pc[k] = (c && ok) ? c[at] : 0.0;
-            px[k] = (x && ok) ? static_cast<double>(x[at]) : 0.0;
+            if constexpr (HX) px[k] = ok ? x[at] : TX(0);
         }
+        const int zo = zi - KL;
+        pb = (b && out && zo >= zs && zo < ze) ? b[gx + static_cast<long>(dm.nx) * gy + sxy * zo] : 0.0;
     };
-    auto stash = [&](const double* pc, const double* px) {
+
+    auto plane = [&](int zi, int buf, double* pc, TX* px, double& pb) {
+        // the tile goes to buffer buf, whose previous readers (plane zi-2)
+        // finished before the last barrier; then plane zi+1's loads go out
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
             const int i = tid + k * NTH;
             if (i < NT) {
-                const int rr = i / TXH, cc = i - rr * TXH;
-                tc[rr][cc] = pc[k];
-                td[rr][cc] = pc[k] - px[k];
+                const int rr = i / kWave, cc = i - rr * kWave;
+                tc[buf][rr][cc] = pc[k];
+                if constexpr (HX) td[buf][rr][cc] = pc[k] - static_cast<double>(px[k]);
             }
         }
-    };
-
-    auto plane = [&](int zi, double* pc, double* px) {
-        // this plane's tile goes to LDS (the previous plane's readers finished
-        // before the last barrier), then the loads of plane zi+2 are issued
-        stash(pc, px);
-        if (zi + 2 < zend) fetch(zi + 2, pc, px);
+        const double bv = pb;
+        if (zi + 1 < zend) fetch(zi + 1, pc, px, pb);
         __syncthreads();
-        for (int rr = wv; rr < TYH; rr += kResRows) {
-            double se = 0.0, sm = 0.0, sk = 0.0;
-#pragma unroll
-            for (int t = 0; t < BW; ++t) {
-                const double cv = tc[rr][lane + t], dv = td[rr][lane + t];
-                se = fma(mxr[t], dv, se);
-                sm = fma(mxr[t], cv, sm);
-                sk = fma(kxr[t], cv, sk);
-            }
-            te[rr][lane] = fma(-dtk, sk, se);
-            tm[rr][lane] = sm;
-        }
-        const double c_here = tc[wv + KL][lane + KL];
-        __syncthreads();
-        double f = 0.0, fk = 0.0, g = 0.0;
+        // Y on the wave's row, all 64 columns
+        double ym = 0.0, yk = 0.0, yd = 0.0;
 #pragma unroll
         for (int t = 0; t < BW; ++t) {
-            const double e = te[wv + t][lane], m = tm[wv + t][lane];
-            f = fma(myr[t], e, f);
-            fk = fma(kyr[t], m, fk);
-            g = fma(myr[t], m, g);
+            const double cv = tc[buf][wv + t][lane];
+            ym = fma(myr[t], cv, ym);
+            yk = fma(kyr[t], cv, yk);
+            if constexpr (HX) yd = fma(myr[t], td[buf][wv + t][lane], yd);
         }
-        f = fma(-dtk, fk, f);
+        const double c_here = tc[buf][wv + KL][lane];
+        sy[wv][0][lane + KL] = ym;
+        sy[wv][1][lane + KL] = yk;
+        if constexpr (HX) sy[wv][2][lane + KL] = yd;
+        wave_lds_sync();
+        // X at the output columns (the halo lanes' values are not used)
+        double g = 0.0, kk = 0.0, e = 0.0;
+#pragma unroll
+        for (int t = 0; t < BW; ++t) {
+            const double m = sy[wv][0][lane + t];
+            g = fma(mxr[t], m, g);
+            kk = fma(kxr[t], m, kk);
+            kk = fma(mxr[t], sy[wv][1][lane + t], kk);
+            if constexpr (HX) e = fma(mxr[t], sy[wv][2][lane + t], e);
+        }
+        if constexpr (HX) e = fma(-dtk, kk, e);
+        else e = fma(-dtk, kk, g);
 #pragma unroll
         for (int t = 0; t < BW - 1; ++t) {
             fw[t] = fw[t + 1];
             gw[t] = gw[t + 1];
         }
-        fw[BW - 1] = f;
+        fw[BW - 1] = e;
         gw[BW - 1] = g;
 #pragma unroll
         for (int t = 0; t < KL; ++t) cw[t] = cw[t + 1];
@@ -582,27 +602,24 @@ __global__ __launch_bounds__(kWave* kResRows) void k_kron_residual(
                 o = fma(mzr[t], fw[t], o);
                 ok = fma(kzr[t], gw[t], ok);
             }
-            o = fma(-dtk, ok, o);
-            if (xin && yin) {
-                const long at = gx + static_cast<long>(dm.nx) * gy + sxy * zo;
-                if (b) o += b[at];
-                r[at] = static_cast<TR>(o);
-            }
+            o = fma(-dtk, ok, o) + bv;
+            if (out) r[gx + static_cast<long>(dm.nx) * gy + sxy * zo] = static_cast<TR>(o);
             if (l2_part) {
                 double mc = 0.0;
 #pragma unroll
                 for (int t = 0; t < BW; ++t) mc = fma(mzr[t], gw[t], mc);
-                l2 = fma(cw[0], mc, l2);
+                l2 = fma(out ? cw[0] : 0.0, mc, l2);
             }
         }
     };
 
-    double pa[NPT], xa[NPT], pb[NPT], xb[NPT];
-    fetch(zs - KL, pa, xa);
-    fetch(zs - KL + 1, pb, xb);
+    double pa[NPT];
+    TX xa[HX ? NPT : 1];
+    double ba = 0.0;
+    fetch(zs - KL, pa, xa, ba);
     for (int zi = zs - KL; zi < zend; zi += 2) {
-        plane(zi, pa, xa);
-        if (zi + 1 < zend) plane(zi + 1, pb, xb);
+        plane(zi, 0, pa, xa, ba);
+        if (zi + 1 < zend) plane(zi + 1, 1, pa, xa, ba);
     }
     if (l2_part) {
 #pragma unroll
@@ -661,37 +678,54 @@ hipError_t kron_solve_p(T* F, const SweepTables<T> sw[3], bool trans, hipStream_
     return hipGetLastError();
 }
 
-// The z segments are sized so that one round of resident workgroups covers
-// the grid (a partly filled second round would double the time).
-ResDims residual_dims(const BandSet& bs, long resident)
+// Number of z segments: a workgroup streams its segment plus 2p halo planes,
+// and the grid runs in rounds of `resident` workgroups, so the time goes as
+// rounds x (segment + 2p); the count minimising that is taken.
+ResDims residual_dims(const BandSet& bs, int ow, int kl, long resident)
 {
     ResDims dm{};
     dm.nx = bs.n[0];
     dm.ny = bs.n[1];
     dm.nz = bs.n[2];
-    dm.xchunks = (dm.nx + kWave - 1) / kWave;
+    dm.xchunks = (dm.nx + ow - 1) / ow;
     dm.yblocks = (dm.ny + kResRows - 1) / kResRows;
     const long tiles = static_cast<long>(dm.xchunks) * dm.yblocks;
-    long nseg = resident / tiles;
-    if (nseg > dm.nz) nseg = dm.nz;
-    if (nseg < 1) nseg = 1;
-    dm.zseg = static_cast<int>((dm.nz + nseg - 1) / nseg);
+    if (resident < 1) resident = 1;
+    long best = -1;
+    dm.zseg = dm.nz;
+    for (int nseg = 1; nseg <= dm.nz && nseg <= 64; ++nseg) {
+        const int zseg = (dm.nz + nseg - 1) / nseg;
+        const long groups = tiles * ((dm.nz + zseg - 1) / zseg);
+        const long cost = ((groups + resident - 1) / resident) * (zseg + 2L * kl);
+        if (best < 0 || cost < best) {
+            best = cost;
+            dm.zseg = zseg;
+        }
+    }
     return dm;
+}
+
+template <int P, typename TX, typename TR, bool HX>
+hipError_t residual_launch(const double* c, const TX* x, const double* b, TR* r, const BandSet& bs, double dtk,
+                           double* l2_part, int part_cap, int* nparts, hipStream_t s)
+{
+    static const int resident =
+        resident_groups(reinterpret_cast<const void*>(&k_kron_residual<P, TX, TR, HX>), kWave * kResRows, 0);
+    const ResDims dm = residual_dims(bs, res_out_width(P - 1), P - 1, resident);
+    const int blocks = dm.xchunks * dm.yblocks * ((dm.nz + dm.zseg - 1) / dm.zseg);
+    *nparts = blocks;
+    if (l2_part && blocks > part_cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_kron_residual<P, TX, TR, HX>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, b, r,
+                       bs.mass[0], bs.stiff[0], bs.mass[1], bs.stiff[1], bs.mass[2], bs.stiff[2], dtk, dm, l2_part);
+    return hipGetLastError();
 }
 
 template <int P, typename TX, typename TR>
 hipError_t residual_p(const double* c, const TX* x, const double* b, TR* r, const BandSet& bs, double dtk,
                       double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
-    static const int resident =
-        resident_groups(reinterpret_cast<const void*>(&k_kron_residual<P, TX, TR>), kWave * kResRows, 0);
-    const ResDims dm = residual_dims(bs, resident);
-    const int blocks = dm.xchunks * dm.yblocks * ((dm.nz + dm.zseg - 1) / dm.zseg);
-    *nparts = blocks;
-    if (l2_part && blocks > part_cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_kron_residual<P, TX, TR>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, b, r,
-                       bs.mass[0], bs.stiff[0], bs.mass[1], bs.stiff[1], bs.mass[2], bs.stiff[2], dtk, dm, l2_part);
-    return hipGetLastError();
+    if (x) return residual_launch<P, TX, TR, true>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
+    return residual_launch<P, TX, TR, false>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
 }
 
 }  // namespace
@@ -713,7 +747,9 @@ template hipError_t launch_kron_solve<float>(int, float*, const SweepTables<floa
 
 long kron_residual_partials_bound(const BandSet& bs)
 {
-    return static_cast<long>((bs.n[0] + kWave - 1) / kWave) * ((bs.n[1] + kResRows - 1) / kResRows) * bs.n[2];
+    // the narrowest output tile (p = 4) gives the most x chunks
+    const int ow = res_out_width(4);
+    return static_cast<long>((bs.n[0] + ow - 1) / ow) * ((bs.n[1] + kResRows - 1) / kResRows) * bs.n[2];
 }
 
 int resident_groups(const void* fn, int threads, size_t lds)
