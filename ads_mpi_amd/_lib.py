@@ -29,6 +29,7 @@ ADS_F0_BUMP3D = 2
 ADS_PREC_FP64 = 0
 ADS_PREC_FP32_IR = 1
 ADS_PREC_FP32_IR_KRON = 2
+ADS_PREC_FP64_KRON = 3
 
 ADS_KERNEL_HEAT_FP64 = 0
 ADS_KERNEL_KRON_RESIDUAL = 1

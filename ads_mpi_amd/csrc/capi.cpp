@@ -232,15 +232,15 @@ ads::BandSet band_set(const ads_ctx* ctx)
     return bs;
 }
 
-// r = b + A c - M x in FP64 (nullptr operands are zero), rounded to FP32;
+// r = b + A c - M x in FP64 (nullptr operands are zero), rounded to TR;
 // with l2_d also l2 = c^T M c -> l2_d.
-template <typename TX>
-int residual(ads_ctx* ctx, const double* c, const TX* x, const double* b, float* r, double dtk, double* l2_d)
+template <typename TX, typename TR = float>
+int residual(ads_ctx* ctx, const double* c, const TX* x, const double* b, TR* r, double dtk, double* l2_d)
 {
     const ads::BandSet bs = band_set(ctx);
     int nres = 0;
-    ADS_HIP(ctx, (ads::launch_kron_residual<TX, float>(ctx->p, c, x, b, r, bs, dtk, l2_d ? ctx->l2_part : nullptr,
-                                                       ctx->l2_cap, &nres, ctx->stream)));
+    ADS_HIP(ctx, (ads::launch_kron_residual<TX, TR>(ctx->p, c, x, b, r, bs, dtk, l2_d ? ctx->l2_part : nullptr,
+                                                    ctx->l2_cap, &nres, ctx->stream)));
     if (l2_d) ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nres, l2_d, ctx->stream));
     return ADS_OK;
 }
@@ -571,10 +571,13 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
 {
     if (check_ctx(ctx) || !c_in_d || !c_out_d) return ADS_ERR_INVALID;
     if (!std::isfinite(dt)) return fail(ctx, ADS_ERR_INVALID, "dt is not finite");
-    if (prec == ADS_PREC_FP64) {
+    if (prec == ADS_PREC_FP64 || prec == ADS_PREC_FP64_KRON) {
         double* target = (c_in_d == c_out_d) ? (c_out_d == ctx->F ? ctx->c : ctx->F) : c_out_d;
         if (target == c_in_d) return fail(ctx, ADS_ERR_INVALID, "internal: buffer aliasing");
-        int rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in_d, target, dt, kappa_d, l2_d);
+        // the Kronecker right-hand side needs a constant diffusivity
+        int rc = (prec == ADS_PREC_FP64_KRON && !kappa_d)
+                     ? residual<double, double>(ctx, c_in_d, nullptr, nullptr, target, dt, l2_d ? l2_d : ctx->l2)
+                     : assemble<double>(ctx, ads::kFormHeat, 0, c_in_d, target, dt, kappa_d, l2_d);
         if (rc) return rc;
         ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, target, ctx->sw64, false, ctx->stream));
         if (target != c_out_d)
@@ -599,7 +602,7 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
     int rc;
     if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
     int nparts = 0;
-    if (prec == ADS_PREC_FP64) {
+    if (prec == ADS_PREC_FP64 || prec == ADS_PREC_FP64_KRON) {
         // mu = M^-T lam_next
         ADS_HIP(ctx, hipMemcpyAsync(ctx->mu, lam_next_d, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice,
                                     ctx->stream));

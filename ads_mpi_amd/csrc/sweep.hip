@@ -23,17 +23,21 @@ constexpr int kWave = 64;
 // banded substitution passes (dgbtrs 'N' and 'T' with dtbsv, restated as
 // streaming recurrences over a window of registers).  KL = p, K = 2p.
 //   NF  L solve with row interchanges, ascending:   in(j+KL+1) -> out(j)
-//   NB  U solve (column form), descending:           in(j-K-1)  -> out(j)
+//   NB  U solve (column form), descending:           in(j-UB-1) -> out(j)
 //   TF  U^T solve (row form), ascending:              in(j)      -> out(j)
 //   TB  L^T solve + interchanges, descending:         in(j)      -> out(j+KL)
+// U has UB = 2p superdiagonals in general (dgbtrf's fill-in from row
+// interchanges).  PIV = false is the instance for factorisations without
+// interchanges (every ipiv(j) = j, the case at all benchmark sizes): there
+// the fill-in entries are exact zeros, so UB = p, and the swap is a no-op.
+// A pass reads NT factor entries per step (load); the callers load them into
+// registers a group of steps ahead of the recurrence.
 // ---------------------------------------------------------------------------
-// PIV = false is the instance for factorisations without row interchanges
-// (every ipiv(j) = j, the case at all benchmark sizes): the swap is a no-op
-// there and the step is straight-line code.
 template <typename T, int KL, bool PIV>
 struct PassNF {
     static constexpr bool kAscending = true;
     static constexpr int kIn = KL + 1, kOut = 0;
+    static constexpr int NT = KL;
     T w[KL + 1];
     template <class LD>
     __device__ void prologue(LD ld, int n)
@@ -41,10 +45,16 @@ struct PassNF {
 #pragma unroll
         for (int t = 0; t <= KL; ++t) w[t] = t < n ? ld(t) : T(0);
     }
-    __device__ T step(int j, T in, const SweepTables<T>& st)
+    __device__ static void load(int j, const T* __restrict__ fwd, const T* __restrict__, const int* __restrict__ piv,
+                                T* tb, int& l)
+    {
+#pragma unroll
+        for (int t = 0; t < KL; ++t) tb[t] = fwd[j * KL + t];
+        l = PIV ? piv[j] : 0;
+    }
+    __device__ T step(T in, const T* tb, int l)
     {
         if constexpr (PIV) {
-            const int l = st.piv[j];
             if (l != 0) {
 #pragma unroll
                 for (int t = 1; t <= KL; ++t)
@@ -57,7 +67,7 @@ struct PassNF {
         }
         const T xj = w[0];
 #pragma unroll
-        for (int t = 1; t <= KL; ++t) w[t] -= st.fwd[j * KL + t - 1] * xj;
+        for (int t = 1; t <= KL; ++t) w[t] -= tb[t - 1] * xj;
 #pragma unroll
         for (int t = 0; t < KL; ++t) w[t] = w[t + 1];
         w[KL] = in;
@@ -72,23 +82,31 @@ struct PassNF {
 template <typename T, int KL, bool PIV>
 struct PassNB {
     static constexpr bool kAscending = false;
-    static constexpr int K = 2 * KL;
-    static constexpr int kIn = -(K + 1), kOut = 0;
-    T v[K + 1];
+    static constexpr int K = 2 * KL, UB = PIV ? K : KL;
+    static constexpr int kIn = -(UB + 1), kOut = 0;
+    static constexpr int NT = UB + 1;
+    T v[UB + 1];
     template <class LD>
     __device__ void prologue(LD ld, int n)
     {
 #pragma unroll
-        for (int t = 0; t <= K; ++t) v[K - t] = (n - 1 - t >= 0) ? ld(n - 1 - t) : T(0);
+        for (int t = 0; t <= UB; ++t) v[UB - t] = (n - 1 - t >= 0) ? ld(n - 1 - t) : T(0);
     }
-    __device__ T step(int j, T in, const SweepTables<T>& st)
+    __device__ static void load(int j, const T* __restrict__, const T* __restrict__ bwd, const int* __restrict__,
+                                T* tb, int& l)
     {
-        const T* b = st.bwd + static_cast<size_t>(j) * (K + 1);
-        const T xj = v[K] * b[0];
+        const T* b = bwd + static_cast<size_t>(j) * (K + 1);
 #pragma unroll
-        for (int t = 1; t <= K; ++t) v[K - t] -= b[t] * xj;
+        for (int t = 0; t < NT; ++t) tb[t] = b[t];
+        l = 0;
+    }
+    __device__ T step(T in, const T* tb, int)
+    {
+        const T xj = v[UB] * tb[0];
 #pragma unroll
-        for (int t = K; t > 0; --t) v[t] = v[t - 1];
+        for (int t = 1; t <= UB; ++t) v[UB - t] -= tb[t] * xj;
+#pragma unroll
+        for (int t = UB; t > 0; --t) v[t] = v[t - 1];
         v[0] = in;
         return xj;
     }
@@ -101,24 +119,32 @@ struct PassNB {
 template <typename T, int KL, bool PIV>
 struct PassTF {
     static constexpr bool kAscending = true;
-    static constexpr int K = 2 * KL;
+    static constexpr int K = 2 * KL, UB = PIV ? K : KL;
     static constexpr int kIn = 0, kOut = 0;
-    T h[K];
+    static constexpr int NT = UB + 1;
+    T h[UB];
     template <class LD>
     __device__ void prologue(LD, int)
     {
 #pragma unroll
-        for (int t = 0; t < K; ++t) h[t] = T(0);
+        for (int t = 0; t < UB; ++t) h[t] = T(0);
     }
-    __device__ T step(int j, T in, const SweepTables<T>& st)
+    __device__ static void load(int j, const T* __restrict__, const T* __restrict__ bwd, const int* __restrict__,
+                                T* tb, int& l)
     {
-        const T* b = st.bwd + static_cast<size_t>(j) * (K + 1);
+        const T* b = bwd + static_cast<size_t>(j) * (K + 1);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) tb[t] = b[t];
+        l = 0;
+    }
+    __device__ T step(T in, const T* tb, int)
+    {
         T s = in;
 #pragma unroll
-        for (int t = K; t >= 1; --t) s -= b[t] * h[t - 1];
-        const T xj = s * b[0];
+        for (int t = UB; t >= 1; --t) s -= tb[t] * h[t - 1];
+        const T xj = s * tb[0];
 #pragma unroll
-        for (int t = K - 1; t > 0; --t) h[t] = h[t - 1];
+        for (int t = UB - 1; t > 0; --t) h[t] = h[t - 1];
         h[0] = xj;
         return xj;
     }
@@ -132,6 +158,7 @@ template <typename T, int KL, bool PIV>
 struct PassTB {
     static constexpr bool kAscending = false;
     static constexpr int kIn = 0, kOut = KL;
+    static constexpr int NT = KL;
     T w[KL + 1];
     template <class LD>
     __device__ void prologue(LD, int)
@@ -139,16 +166,22 @@ struct PassTB {
 #pragma unroll
         for (int t = 0; t <= KL; ++t) w[t] = T(0);
     }
-    __device__ T step(int j, T in, const SweepTables<T>& st)
+    __device__ static void load(int j, const T* __restrict__ fwd, const T* __restrict__, const int* __restrict__ piv,
+                                T* tb, int& l)
+    {
+#pragma unroll
+        for (int t = 0; t < KL; ++t) tb[t] = fwd[j * KL + t];
+        l = PIV ? piv[j] : 0;
+    }
+    __device__ T step(T in, const T* tb, int l)
     {
 #pragma unroll
         for (int t = KL; t > 0; --t) w[t] = w[t - 1];
         T s = T(0);
 #pragma unroll
-        for (int t = 1; t <= KL; ++t) s += w[t] * st.fwd[j * KL + t - 1];
+        for (int t = 1; t <= KL; ++t) s += w[t] * tb[t - 1];
         w[0] = in - s;
         if constexpr (PIV) {
-            const int l = st.piv[j];
             if (l != 0) {
 #pragma unroll
                 for (int t = 1; t <= KL; ++t)
@@ -217,13 +250,28 @@ __device__ __forceinline__ bool chunk_inside(int ch, int nch, int n, int off)
     return lo >= 0 && hi < n && lo + off >= 0 && hi + off < n;
 }
 
+// Where the line data sit in LDS beside the factor tables (the row sweep's
+// tiles), the factor entries of kGroup consecutive steps are loaded into
+// registers before the recurrence runs over them, so their loads do not
+// queue behind the stores of the steps before.  Lines in global memory load
+// them step by step (group 1) and keep registers for the data prefetch.
+constexpr int kGroup = 8;
+
+template <class PS, typename T, int G>
+struct GroupTabs {
+    T tb[G][PS::NT];
+    int l[G];
+};
+
 // One lane per line; element k of the line at base[k*stride].  Inputs are
 // fetched CH at a time one chunk ahead of the recurrence (double buffer in
 // VGPRs), so every lane keeps CH loads in flight.
-template <typename T, int KL, template <typename, int, bool> class PASS, int CH, bool PIV>
-__device__ __forceinline__ void run_line(T* __restrict__ base, long stride, int n, const SweepTables<T>& st)
+template <typename T, int KL, template <typename, int, bool> class PASS, int CH, bool PIV, int G = 1>
+__device__ __forceinline__ void run_line(T* __restrict__ base, long stride, int n, const T* __restrict__ fwd,
+                                         const T* __restrict__ bwd, const int* __restrict__ piv)
 {
     using PS = PASS<T, KL, PIV>;
+    static_assert(CH % G == 0, "chunk of whole groups");
     PS ps;
     ps.prologue([&](int k) { return base[static_cast<long>(k) * stride]; }, n);
     const int nch = (n + CH - 1) / CH;
@@ -248,18 +296,34 @@ __device__ __forceinline__ void run_line(T* __restrict__ base, long stride, int 
         fetch(ch + 1, nxt);
         if (chunk_inside<PS, CH>(ch, nch, n, PS::kOut)) {
 #pragma unroll
-            for (int i = 0; i < CH; ++i) {
-                const int j = chunk_step<PS, CH>(ch, i, n);
-                base[static_cast<long>(j + PS::kOut) * stride] = ps.step(j, cur[i], st);
+            for (int g = 0; g < CH; g += G) {
+                GroupTabs<PS, T, G> gt;
+#pragma unroll
+                for (int i = 0; i < G; ++i)
+                    PS::load(chunk_step<PS, CH>(ch, g + i, n), fwd, bwd, piv, gt.tb[i], gt.l[i]);
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int j = chunk_step<PS, CH>(ch, g + i, n);
+                    base[static_cast<long>(j + PS::kOut) * stride] = ps.step(cur[g + i], gt.tb[i], gt.l[i]);
+                }
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < CH; ++i) {
-                const int j = chunk_step<PS, CH>(ch, i, n);
-                if (j >= 0 && j < n) {
-                    const T o = ps.step(j, cur[i], st);
-                    const int k = j + PS::kOut;
-                    if (k < n) base[static_cast<long>(k) * stride] = o;
+            for (int g = 0; g < CH; g += G) {
+                GroupTabs<PS, T, G> gt;
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int j = chunk_step<PS, CH>(ch, g + i, n);
+                    if (j >= 0 && j < n) PS::load(j, fwd, bwd, piv, gt.tb[i], gt.l[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int j = chunk_step<PS, CH>(ch, g + i, n);
+                    if (j >= 0 && j < n) {
+                        const T o = ps.step(cur[g + i], gt.tb[i], gt.l[i]);
+                        const int k = j + PS::kOut;
+                        if (k < n) base[static_cast<long>(k) * stride] = o;
+                    }
                 }
             }
         }
@@ -290,11 +354,11 @@ __global__ __launch_bounds__(256) void k_sweep_lines(T* __restrict__ data, int n
     T* base = data + i + static_cast<long>(j) * stride_j;
     constexpr int CH = LineChunk<T>::value;
     if constexpr (!TRANS) {
-        run_line<T, KL, PassNF, CH, PIV>(base, stride_k, st.n, st);
-        run_line<T, KL, PassNB, CH, PIV>(base, stride_k, st.n, st);
+        run_line<T, KL, PassNF, CH, PIV>(base, stride_k, st.n, st.fwd, st.bwd, st.piv);
+        run_line<T, KL, PassNB, CH, PIV>(base, stride_k, st.n, st.fwd, st.bwd, st.piv);
     } else {
-        run_line<T, KL, PassTF, CH, PIV>(base, stride_k, st.n, st);
-        run_line<T, KL, PassTB, CH, PIV>(base, stride_k, st.n, st);
+        run_line<T, KL, PassTF, CH, PIV>(base, stride_k, st.n, st.fwd, st.bwd, st.piv);
+        run_line<T, KL, PassTB, CH, PIV>(base, stride_k, st.n, st.fwd, st.bwd, st.piv);
     }
 }
 
@@ -306,23 +370,28 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kRowTile = 16;   // columns per LDS tile
-constexpr int kRowWaves = 4;   // at most this many waves per row-sweep workgroup (one table copy)
+// columns per LDS tile of the streaming row sweep
+template <typename T>
+struct RowTile {
+    static constexpr int value = 16;
+};
+constexpr int kRowWaves = 4;  // at most this many waves per row-sweep workgroup (one table copy)
 
 // Contiguous rows (the x direction): a wave owns 64 rows and moves them
-// through its own LDS tile, kRowTile columns at a time, so every global
+// through its own LDS tile, RowTile<T> columns at a time, so every global
 // access is row-contiguous; the next tile is fetched into VGPRs while the
 // current one is solved in place.
 template <typename T, int KL, template <typename, int, bool> class PASS, bool PIV>
-__device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int row0, int lane,
-                                         const SweepTables<T>& st, T* tile)
+__device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int row0, int lane, int n,
+                                         const T* __restrict__ fwd, const T* __restrict__ bwd,
+                                         const int* __restrict__ piv, T* __restrict__ tile)
 {
-    constexpr int CH = kRowTile;
+    constexpr int CH = RowTile<T>::value;
     constexpr int LDT = CH + 1;
     constexpr int RPI = kWave / CH;      // rows per cooperative instruction
     constexpr int NIT = kWave / RPI;     // cooperative instructions per tile
     using PS = PASS<T, KL, PIV>;
-    const int n = st.n;
+    static_assert(CH % kGroup == 0, "tile of whole groups");
     const long rowlen = n;
     const int myrow = row0 + lane;
     const bool active = myrow < nrows;
@@ -355,16 +424,35 @@ __device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int ro
         wave_lds_sync();
         if (jlo >= 0 && jlo + CH <= n) {
 #pragma unroll
-            for (int i = 0; i < CH; ++i) {
-                const int c = PS::kAscending ? i : CH - 1 - i;
-                tile[lane * LDT + c] = ps.step(jlo + c, tile[lane * LDT + c], st);
+            for (int g = 0; g < CH; g += kGroup) {
+                GroupTabs<PS, T, kGroup> gt;
+#pragma unroll
+                for (int i = 0; i < kGroup; ++i) {
+                    const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
+                    PS::load(jlo + c, fwd, bwd, piv, gt.tb[i], gt.l[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < kGroup; ++i) {
+                    const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
+                    tile[lane * LDT + c] = ps.step(tile[lane * LDT + c], gt.tb[i], gt.l[i]);
+                }
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < CH; ++i) {
-                const int c = PS::kAscending ? i : CH - 1 - i;
-                const int j = jlo + c;
-                if (j >= 0 && j < n) tile[lane * LDT + c] = ps.step(j, tile[lane * LDT + c], st);
+            for (int g = 0; g < CH; g += kGroup) {
+                GroupTabs<PS, T, kGroup> gt;
+#pragma unroll
+                for (int i = 0; i < kGroup; ++i) {
+                    const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
+                    const int j = jlo + c;
+                    if (j >= 0 && j < n) PS::load(j, fwd, bwd, piv, gt.tb[i], gt.l[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < kGroup; ++i) {
+                    const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
+                    const int j = jlo + c;
+                    if (j >= 0 && j < n) tile[lane * LDT + c] = ps.step(tile[lane * LDT + c], gt.tb[i], gt.l[i]);
+                }
             }
         }
         wave_lds_sync();
@@ -398,30 +486,32 @@ constexpr size_t kLdsPerGroup = 65536;
 template <typename T, int KL>
 int row_waves(int n)
 {
-    const size_t tile = static_cast<size_t>(kWave) * (kRowTile + 1) * sizeof(T);
+    const size_t tile = static_cast<size_t>(kWave) * (RowTile<T>::value + 1) * sizeof(T);
     const size_t tb = row_table_bytes<T, KL>(n);
     if (tb + tile > kLdsPerGroup) return 0;
     const size_t w = (kLdsPerGroup - tb) / tile;
     return static_cast<int>(w < static_cast<size_t>(kRowWaves) ? w : kRowWaves);
 }
 
+// two workgroups per CU: about 263 of them cover 256^2 rows in one round
 template <typename T, int KL, bool TRANS, bool PIV>
-__global__ __launch_bounds__(kWave* kRowWaves) void k_sweep_rows(T* __restrict__ data, int nrows,
+__global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restrict__ data, int nrows,
                                                                  SweepTables<T> gst)
 {
     extern __shared__ unsigned char smem[];
     const SweepTables<T> st = stage_tables<T, KL>(gst, smem);
     const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
     // each wave's tile follows the tables
-    T* tile = reinterpret_cast<T*>(smem + row_table_bytes<T, KL>(gst.n)) + static_cast<size_t>(wv) * kWave * (kRowTile + 1);
+    T* tile = reinterpret_cast<T*>(smem + row_table_bytes<T, KL>(gst.n)) +
+              static_cast<size_t>(wv) * kWave * (RowTile<T>::value + 1);
     const int row0 = (blockIdx.x * nw + wv) * kWave;
     if (row0 >= nrows) return;
     if constexpr (!TRANS) {
-        run_rows<T, KL, PassNF, PIV>(data, nrows, row0, lane, st, tile);
-        run_rows<T, KL, PassNB, PIV>(data, nrows, row0, lane, st, tile);
+        run_rows<T, KL, PassNF, PIV>(data, nrows, row0, lane, st.n, st.fwd, st.bwd, st.piv, tile);
+        run_rows<T, KL, PassNB, PIV>(data, nrows, row0, lane, st.n, st.fwd, st.bwd, st.piv, tile);
     } else {
-        run_rows<T, KL, PassTF, PIV>(data, nrows, row0, lane, st, tile);
-        run_rows<T, KL, PassTB, PIV>(data, nrows, row0, lane, st, tile);
+        run_rows<T, KL, PassTF, PIV>(data, nrows, row0, lane, st.n, st.fwd, st.bwd, st.piv, tile);
+        run_rows<T, KL, PassTB, PIV>(data, nrows, row0, lane, st.n, st.fwd, st.bwd, st.piv, tile);
     }
 }
 
@@ -663,7 +753,8 @@ hipError_t kron_solve_p(T* F, const SweepTables<T> sw[3], bool trans, hipStream_
 {
     const int nx = sw[0].n, ny = sw[1].n, nz = sw[2].n;
     const int rw = row_waves<T, KL>(nx);
-    const size_t bx = row_table_bytes<T, KL>(nx) + static_cast<size_t>(rw) * kWave * (kRowTile + 1) * sizeof(T);
+    const size_t bx =
+        row_table_bytes<T, KL>(nx) + static_cast<size_t>(rw) * kWave * (RowTile<T>::value + 1) * sizeof(T);
     const size_t by = sweep_table_bytes<T, KL>(ny), bz = sweep_table_bytes<T, KL>(nz);
     if (rw < 1 || by > kLdsPerGroup || bz > kLdsPerGroup) return hipErrorInvalidValue;  // line too long for the LDS tables
     // the pivot-free instance when no direction's factorisation has interchanges

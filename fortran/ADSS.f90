@@ -19,7 +19,8 @@ module ADSS
    public :: initialize, Step, MultiStep, Cleanup, PrintSolution
    public :: RegisterHeatForm, RegisterProjectionForm, SyncSolution, SetDiffusivity
    public :: ads_sync_every_step
-   public :: ADS_F0_CONST, ADS_F0_SINPROD, ADS_F0_BUMP3D, ADS_PREC_FP64, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_KRON
+   public :: ADS_F0_CONST, ADS_F0_SINPROD, ADS_F0_BUMP3D, ADS_PREC_FP64, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_KRON, &
+             ADS_PREC_FP64_KRON
 
    integer, parameter :: FORM_HEAT = 1, FORM_PROJECTION = 2, MAXFORMS = 16
 

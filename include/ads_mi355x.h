@@ -41,6 +41,7 @@ extern "C" {
 #define ADS_PREC_FP64 0    /* FP64 assembly and sweeps                            */
 #define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 residual refinement */
 #define ADS_PREC_FP32_IR_KRON 2 /* FP32 sweeps on FP64 residuals from a zero start */
+#define ADS_PREC_FP64_KRON 3    /* FP64 sweeps on the fused FP64 Kronecker right-hand side */
 
 /* kernels for ads_time_kernel */
 #define ADS_KERNEL_HEAT_FP64 0      /* heat RHS assembly, FP64 */
@@ -111,6 +112,10 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  *   corrections, each an FP64 residual and FP32 sweeps.
  * prec = ADS_PREC_FP32_IR_KRON: refine = 1..4 FP32 sweeps in all, on FP64
  *   residuals from a zero start (the first residual is the right-hand side).
+ * prec = ADS_PREC_FP64_KRON: FP64 throughout; for a constant diffusivity the
+ *   right-hand side is the Kronecker operator (Mz(x)My(x)Mx - dt K) c applied
+ *   with the 1-D matrices (equal to the Gauss-point assembly, whose sum over
+ *   elements and points factorises per direction), else the element assembly.
  * The FP64 residual is the Kronecker operator A c - M x for a constant
  * diffusivity, and b - M x with b = A(kappa) c from the FP64 element
  * assembly for a per-element one.  l2norm (may be NULL: no host sync)
@@ -157,7 +162,7 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
  * gradients).  Given c_n and lam_next = dL/dc_{n+1}:
  *   mu = M^-T lam_next,  lam_out = A(kappa) mu = dL/dc_n,
  *   dkappa[e] += -dt sum_{q in e} J W grad c_n . grad mu   (if dkappa_d != NULL).
- * prec = ADS_PREC_FP64: all in FP64 (refine ignored).  ADS_PREC_FP32_IR or
+ * prec = ADS_PREC_FP64 or ADS_PREC_FP64_KRON: all in FP64 (refine ignored).  ADS_PREC_FP32_IR or
  * ADS_PREC_FP32_IR_KRON: mu by FP32 transposed sweeps with refine-1 FP64
  * mass-residual corrections (refine = 1..4), then the assembly applied to
  * mu and the diffusivity gradient in FP32 arithmetic (dkappa accumulated in
