@@ -530,15 +530,42 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 //     double-buffered LDS tile (one barrier per plane); the loads of the
 //     next plane are in flight meanwhile;
 //   * the Y stage runs on all 64 columns of the wave's row (the x halo costs
-//     2p idle lanes instead of recomputing halo rows), its results go to a
-//     row of LDS private to the wave, and the X stage reads its 2p+1
-//     neighbours there;
+//     2p idle lanes instead of recomputing halo rows), and the X stage takes
+//     its 2p+1 neighbours from the lanes beside it: through a row of LDS
+//     private to the wave for the two Y results of the first pass, by DPP
+//     wave shifts for the three of the pass with x (there the LDS traffic
+//     limits, measured 187 against 212 us at 259^3);
 //   * the Z stage keeps the last 2p+1 X results of the lane in registers.
 // About 56 (70 with x) multiply-adds per DOF; HBM traffic about one read of
 // c (and x, b) and one write of r per DOF.
 // Row tables hold A(i, i-KL+t), t = 0..2KL, zero outside the matrix.
 // ---------------------------------------------------------------------------
 constexpr int kResRows = 8;
+
+// Lane i receives lane i+1's value (CTRL 0x130, wave_shl:1) or lane i-1's
+// (0x138, wave_shr:1); the lane at the wave's edge keeps its own (the
+// residual does not use the edge lanes' results).
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi2) << 32) | static_cast<unsigned>(lo2));
+}
+
+// v[t] = the value of lane (lane - KL + t), t = 0..2 KL
+template <int KL>
+__device__ __forceinline__ void lane_window(double own, double* v)
+{
+    v[KL] = own;
+#pragma unroll
+    for (int k = 1; k <= KL; ++k) {
+        v[KL + k] = wave_shift<0x130>(v[KL + k - 1]);
+        v[KL - k] = wave_shift<0x138>(v[KL - k + 1]);
+    }
+}
 
 // output columns per residual workgroup: the wave's width less the x halos
 __host__ __device__ constexpr int res_out_width(int p)
@@ -572,10 +599,9 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
     constexpr int KL = P - 1, BW = 2 * KL + 1;
     constexpr int OW = res_out_width(KL);
     constexpr int TYH = kResRows + 2 * KL, NTH = kWave * kResRows;
-    constexpr int NY = HX ? 3 : 2;  // Y results: ym, yk (, yd)
     __shared__ double tc[2][TYH][kWave];                         // c tiles, rows y0-KL..
     __shared__ double td[HX ? 2 : 1][HX ? TYH : 1][kWave];       // c - x tiles
-    __shared__ double sy[kResRows][NY][kWave + 2 * KL];          // Y results, column lane + KL
+    __shared__ double sy[HX ? 1 : kResRows][2][HX ? 1 : kWave + 2 * KL];  // Y results, column lane + KL
     __shared__ double l2w[kResRows];
 
     const int tid = threadIdx.x;
@@ -656,19 +682,31 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
             if constexpr (HX) yd = fma(myr[t], td[buf][wv + t][lane], yd);
         }
         const double c_here = tc[buf][wv + KL][lane];
-        sy[wv][0][lane + KL] = ym;
-        sy[wv][1][lane + KL] = yk;
-        if constexpr (HX) sy[wv][2][lane + KL] = yd;
-        wave_lds_sync();
         // X at the output columns (the halo lanes' values are not used)
         double g = 0.0, kk = 0.0, e = 0.0;
+        if constexpr (HX) {
+            double vm[BW], vk[BW], vd[BW];
+            lane_window<KL>(ym, vm);
+            lane_window<KL>(yk, vk);
+            lane_window<KL>(yd, vd);
 #pragma unroll
-        for (int t = 0; t < BW; ++t) {
-            const double m = sy[wv][0][lane + t];
-            g = fma(mxr[t], m, g);
-            kk = fma(kxr[t], m, kk);
-            kk = fma(mxr[t], sy[wv][1][lane + t], kk);
-            if constexpr (HX) e = fma(mxr[t], sy[wv][2][lane + t], e);
+            for (int t = 0; t < BW; ++t) {
+                g = fma(mxr[t], vm[t], g);
+                kk = fma(kxr[t], vm[t], kk);
+                kk = fma(mxr[t], vk[t], kk);
+                e = fma(mxr[t], vd[t], e);
+            }
+        } else {
+            sy[wv][0][lane + KL] = ym;
+            sy[wv][1][lane + KL] = yk;
+            wave_lds_sync();
+#pragma unroll
+            for (int t = 0; t < BW; ++t) {
+                const double m = sy[wv][0][lane + t];
+                g = fma(mxr[t], m, g);
+                kk = fma(kxr[t], m, kk);
+                kk = fma(mxr[t], sy[wv][1][lane + t], kk);
+            }
         }
         if constexpr (HX) e = fma(-dtk, kk, e);
         else e = fma(-dtk, kk, g);

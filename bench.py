@@ -13,6 +13,10 @@ runs on, bracketed by device synchronisation).  Alongside it:
     same state, its speed-up and its error against the FP64 step;
   * ``fp32_ir_kron``: the precision form without element assembly (two passes
     of FP64 Kronecker residual + FP32 sweeps from a zero start; constant kappa);
+  * ``fp64_kron``: FP64 throughout, the right-hand side from the fused
+    Kronecker operator of the 1-D matrices instead of the element assembly
+    (exact for a constant kappa, where the Gauss sum over elements and points
+    factorises per direction) + the same FP64 sweeps;
   * ``grad_fwd``: wall ratio of a checkpointed gradient (recompute + adjoint)
     to the forward rollout at 96^3 elements, p = 2, 50 steps, checkpoint 10;
   * ``roofline``: the heat assembly kernel (the step's dominant kernel) timed
@@ -164,7 +168,9 @@ def main():
             ("fp32_ir", _lib.ADS_PREC_FP32_IR, 1,
              "FP32 element assembly + FP32 sweeps, then 1 FP64 Kronecker-residual refinement"),
             ("fp32_ir_kron", _lib.ADS_PREC_FP32_IR_KRON, 2,
-             "zero start, 2 passes of fused FP64 Kronecker residual + FP32 sweeps (no element assembly)")):
+             "zero start, 2 passes of fused FP64 Kronecker residual + FP32 sweeps (no element assembly)"),
+            ("fp64_kron", _lib.ADS_PREC_FP64_KRON, 0,
+             "FP64: fused Kronecker right-hand side (1-D Gauss matrices, constant kappa) + FP64 sweeps")):
         _lib.check(L.ads_step_d(h, vp(cdev), vp(got), None, dt, prec, refine, None), h)
         torch.cuda.synchronize()
         rel_l2 = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
@@ -255,6 +261,7 @@ def main():
         "l2norm": l2.value,
         "fp32_ir": forms["fp32_ir"],
         "fp32_ir_kron": forms["fp32_ir_kron"],
+        "fp64_kron": forms["fp64_kron"],
         "grad_fwd": grad,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
