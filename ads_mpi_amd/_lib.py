@@ -95,6 +95,12 @@ def load(path: str | None = None):
     L.ads_host_grid_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        _ip, _dp]
     L.ads_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
+    L.ads_oil_setup.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
+                                ctypes.c_double * 6]
+    L.ads_oil_project.argtypes = [_vp, _dp]
+    L.ads_oil_step.argtypes = [_vp, ctypes.c_double, _dp, _dp]
+    L.ads_oil_get_kq.argtypes = [_vp, _dp]
+    L.ads_oil_curves.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
     L.ads_fill_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_fill_lognormal.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_host_basis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _ip, _dp, _dp, _dp, _dp]

@@ -37,7 +37,7 @@ from ._lib import (ADS_F0_BUMP3D, ADS_F0_CONST, ADS_F0_SINPROD, ADS_PREC_FP32_IR
 
 __all__ = [
     "ADS_setup", "ADS_compute_data", "initialize", "Step", "MultiStep", "Cleanup", "Form3DRHS", "FormUn",
-    "ComputeMatrix", "HeatRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
+    "ComputeMatrix", "HeatRHS", "OilRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
     "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR", "ADS_PREC_FP32_IR_KRON", "ADS_PREC_FP64_KRON", "PlotParams", "EvalGrid", "VtkOutput",
     "PrintSolution",
 ]
@@ -148,6 +148,54 @@ class HeatRHS:
         self.t = 0.0
 
 
+class OilRHS:
+    """problems/oil/RHS_eq.F90:76-91 with input_data.F90 as a device form:
+    t > 0: ret = J W (v u + Dt (-kq exp(mi u) grad u . grad v + v (pump -
+    max(0, drain u)))), l2norm = J W v u^2; t == 0: the projection of
+    initial_state.  The curves (cN x cL points), pumps and drains are inputs
+    (the reference draws its curves from an unseeded random_number and reads
+    pumps and drains from the command line).  ``drained`` is the running total
+    of J W v Dt vdrain over the steps, as the reference's module variable."""
+
+    def __init__(self, Dt, cx, cy, cz, cN, cL, pumps=(), drains=(), mi=10.0, kq_min=1.0, kq_max=1000.0,
+                 radius=0.15, pump_strength=1.0, drain_strength=1.0):
+        self.Dt = float(Dt)
+        self.cx = np.ascontiguousarray(cx, dtype=np.float64).ravel()
+        self.cy = np.ascontiguousarray(cy, dtype=np.float64).ravel()
+        self.cz = np.ascontiguousarray(cz, dtype=np.float64).ravel()
+        self.cN, self.cL = int(cN), int(cL)
+        self.pumps = np.ascontiguousarray(np.reshape(pumps, (-1, 3)), dtype=np.float64)
+        self.drains = np.ascontiguousarray(np.reshape(drains, (-1, 3)), dtype=np.float64)
+        self.params = (float(mi), float(kq_min), float(kq_max), float(radius), float(pump_strength),
+                       float(drain_strength))
+        self.t = 0.0
+        self.drained = 0.0
+        self._ready_on = None
+
+    @classmethod
+    def with_random_curves(cls, Dt, seed, cN=30, cL=16, **kw):
+        """Curves by InitInputData's random walk (input_data.F90:139-165) with
+        a seeded SplitMix64 stream (ads_oil_curves)."""
+        L = _lib.load()
+        cx, cy, cz = (np.zeros(cN * cL) for _ in range(3))
+        check(L.ads_oil_curves(seed, cN, cL, _d(cx), _d(cy), _d(cz)))
+        return cls(Dt, cx, cy, cz, cN, cL, **kw)
+
+    def setup(self, ads_data):
+        """InitInputData + PrecomputeKq on the device (ads_oil_setup)."""
+        L = _lib.load()
+        h = ads_data.handle
+        if self._ready_on is h:
+            return
+        pm = ctypes.c_double * 6
+        np_ = len(self.pumps)
+        nd = len(self.drains)
+        check(L.ads_oil_setup(h, _d(self.cx), _d(self.cy), _d(self.cz), self.cN, self.cL,
+                              _d(self.pumps) if np_ else None, np_, _d(self.drains) if nd else None, nd,
+                              pm(*self.params)), h)
+        self._ready_on = h
+
+
 class L2ProjectionRHS:
     """problems/L2Projection/RHS_eq.F90:34-71: f = 1."""
 
@@ -219,9 +267,19 @@ def Step(iter, RHS_fun, ads, ads_data):
     if isinstance(RHS_fun, L2ProjectionRHS):
         check(L.ads_project(h, ADS_F0_CONST, ctypes.byref(out)), h)
         return out.value
+    if isinstance(RHS_fun, OilRHS):
+        RHS_fun.setup(ads_data)
+        if RHS_fun.t == 0.0:
+            check(L.ads_oil_project(h, ctypes.byref(out)), h)
+        else:
+            dr = ctypes.c_double(0.0)
+            check(L.ads_oil_step(h, RHS_fun.Dt, ctypes.byref(out), ctypes.byref(dr)), h)
+            RHS_fun.drained += dr.value
+        ads_data.t = RHS_fun.t
+        return out.value
     if not isinstance(RHS_fun, HeatRHS):
         raise AdsError(_lib.ADS_ERR_UNSUPPORTED,
-                       "RHS_fun is not a registered device form (HeatRHS, L2ProjectionRHS)")
+                       "RHS_fun is not a registered device form (HeatRHS, OilRHS, L2ProjectionRHS)")
     if RHS_fun.t == 0.0:
         check(L.ads_project(h, RHS_fun.f0, ctypes.byref(out)), h)
     else:

@@ -444,6 +444,298 @@ __global__ __launch_bounds__(kWave) void k_project(T* __restrict__ F, const T* _
     if (lane == 0) l2_part[blockIdx.x] = l2d;
 }
 
+// ---------------------------------------------------------------------------
+// The oil problem (problems/oil): integrand at t > 0 (RHS_eq.F90:76-87)
+//   J W (v u + Dt (-kq(X) exp(mi u) grad u . grad v + v (pump(X) - max(0, drain(X) u))))
+// It is nonlinear in u, so there are no element matrices: u and grad u are
+// evaluated at the Gauss points (FormUn, sum-factorised x, y, z), the point
+// weights of v and of grad v formed, and contracted back z, y, x into the
+// right-hand-side ring (as k_project).  kq is cached per point (CacheKqValues,
+// input_data.F90:333-380).  The distance and kq arithmetic runs without
+// contraction, in the reference's order, so the cache matches the oracle
+// bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double oil_falloff(double r, double R, double t)
+{
+#pragma clang fp contract(off)
+    if (t < r) return 1.0;
+    if (t > R) return 0.0;
+    const double h = (t - r) / (R - r);
+    const double g = (h - 1.0) * (h + 1.0);
+    return g * g;
+}
+
+// squared distance from the curves (dist_from_curves / dist_from_segment,
+// input_data.F90:171-238), at most 1e3
+__device__ double oil_dist_curves(const OilDev& o, double px, double py, double pz)
+{
+#pragma clang fp contract(off)
+    const int npt = o.cN * o.cL;
+    const double* __restrict__ cx = o.curves;
+    const double* __restrict__ cy = o.curves + npt;
+    const double* __restrict__ cz = o.curves + 2 * npt;
+    double fval = 1e3;
+    for (int i = 0; i < o.cN; ++i)
+        for (int j = 1; j < o.cL; ++j) {
+            const int k = i * o.cL + j;
+            const double ax = cx[k - 1], ay = cy[k - 1], az = cz[k - 1];
+            const double dx = cx[k] - ax, dy = cy[k] - ay, dz = cz[k] - az;
+            const double qx = px - ax, qy = py - ay, qz = pz - az;
+            const double dot = dx * qx + dy * qy + dz * qz;
+            const double len2 = dx * dx + dy * dy + dz * dz;
+            const double proj = dot / len2;
+            double xx, yy, zz;
+            if (proj < 0) {
+                xx = ax; yy = ay; zz = az;
+            } else if (proj > 1) {
+                xx = cx[k]; yy = cy[k]; zz = cz[k];
+            } else {
+                xx = ax + proj * dx; yy = ay + proj * dy; zz = az + proj * dz;
+            }
+            const double ex = px - xx, ey = py - yy, ez = pz - zz;
+            const double d = ex * ex + ey * ey + ez * ez;
+            if (d < fval) fval = d;
+        }
+    return fval;
+}
+
+// sum over the sources at (x, y, z) of strength * falloff(0, radius, |q - X|)
+__device__ __forceinline__ double oil_sources(const double* __restrict__ q, int nq, double strength, double radius,
+                                              double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    double f = 0.0;
+    for (int i = 0; i < nq; ++i) {
+        const double dx = q[3 * i] - x, dy = q[3 * i + 1] - y, dz = q[3 * i + 2] - z;
+        f = f + strength * oil_falloff(0.0, radius, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    return f;
+}
+
+// initial_state (input_data.F90:317-329): 0.1 lerp(falloff(0, .1, dist), 0, 1) bump3d(.2, .6, X)
+__device__ double oil_initial(const OilDev& o, double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    const double dist = sqrt(oil_dist_curves(o, x, y, z));
+    const double bx = x - 0.5, by = y - 0.5, bz = z - 0.5;
+    const double b = oil_falloff(0.2 / 2.0, 0.6 / 2.0, sqrt(bx * bx + by * by + bz * bz));
+    const double t = oil_falloff(0.0, 0.1, dist);
+    return 0.1 * ((1.0 - t) * 0.0 + t * 1.0) * b;
+}
+
+template <int Q>
+__global__ __launch_bounds__(256) void k_oil_kq(const double* __restrict__ ptsx, const double* __restrict__ ptsy,
+                                                const double* __restrict__ ptsz, int nex, int ney, int nez, OilDev o,
+                                                double* __restrict__ kq)
+{
+#pragma clang fp contract(off)
+    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    const long n = static_cast<long>(nex) * ney * nez * Q * Q * Q;
+    if (i >= n) return;
+    const int kx = static_cast<int>(i % Q), ky = static_cast<int>((i / Q) % Q), kz = static_cast<int>((i / (Q * Q)) % Q);
+    const long e = i / (Q * Q * Q);
+    const int ex = static_cast<int>(e % nex), ey = static_cast<int>((e / nex) % ney), ez = static_cast<int>(e / (static_cast<long>(nex) * ney));
+    const double dist = sqrt(oil_dist_curves(o, ptsx[ex * Q + kx], ptsy[ey * Q + ky], ptsz[ez * Q + kz]));
+    const double t = oil_falloff(0.0, 0.06, dist);
+    kq[i] = (1.0 - t) * o.kq_min + t * o.kq_max;
+}
+
+template <int P, bool TZERO>
+__global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, double* __restrict__ F,
+                                               const double* __restrict__ tabx, const double* __restrict__ taby,
+                                               const double* __restrict__ tabz, const double* __restrict__ jacx,
+                                               const double* __restrict__ jacy, const double* __restrict__ jacz,
+                                               const double* __restrict__ Wx, const double* __restrict__ Wy,
+                                               const double* __restrict__ Wz, const double* __restrict__ ptsx,
+                                               const double* __restrict__ ptsy, const double* __restrict__ ptsz,
+                                               AsmDims dm, OilDev o, const double* __restrict__ kq, double dt,
+                                               double* __restrict__ l2_part, double* __restrict__ dr_part)
+{
+    constexpr int Q = P, KL = P - 1, W = kWave + KL;
+    __shared__ double fs[P][P][W];  // right-hand-side planes, ring by z
+
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int xc = tile % dm.xchunks;
+    const int ey = (tile / dm.xchunks) % dm.ney;
+    const int seg = tile / (dm.xchunks * dm.ney);
+    const int ex0 = xc * kWave;
+    const int ex = ex0 + lane;
+    const int ez0 = seg * dm.zseg;
+    const int ez1 = min(ez0 + dm.zseg, dm.nez);
+    const bool active = ex < dm.nex;
+    const int exc = active ? ex : dm.nex - 1;
+    const int sx = dm.sx;
+    const long plane = static_cast<long>(sx) * dm.sy;
+    const int ncol = min(W, sx - ex0);
+
+    const double* __restrict__ Nx = tabx + static_cast<size_t>(exc) * Q * P * 2;
+    const double* __restrict__ Ny = taby + static_cast<size_t>(ey) * Q * P * 2;
+    const double Jxy = jacx[exc] * jacy[ey];
+
+    for (int i = lane; i < P * P * W; i += kWave) (&fs[0][0][0])[i] = 0.0;
+    auto flush_plane = [&](int z) {
+        const int slot = z % P;
+        double* dst = F + ex0 + static_cast<long>(sx) * ey + plane * z;
+        for (int i = lane; i < P * W; i += kWave) {
+            const int ay = i / W, col = i - ay * W;
+            if (col < ncol) atomic_add(&dst[col + static_cast<long>(sx) * ay], fs[slot][ay][col]);
+            fs[slot][ay][col] = 0.0;
+        }
+    };
+    __syncthreads();
+
+    double l2 = 0.0, dr = 0.0;
+    for (int ez = ez0; ez < ez1; ++ez) {
+        const double* __restrict__ Nz = tabz + static_cast<size_t>(ez) * Q * P * 2;
+        const double J = Jxy * jacz[ez];
+        const int s0 = ez % P;
+        int slot[P];
+#pragma unroll
+        for (int az = 0; az < P; ++az) slot[az] = (s0 + az) % P;
+        const long e = exc + static_cast<long>(dm.nex) * (ey + static_cast<long>(dm.ney) * ez);
+        // the element's coefficients: first DOF of element e is e (open knots)
+        const double* __restrict__ ce = c ? c + exc + static_cast<long>(sx) * ey + plane * ez : nullptr;
+
+#pragma unroll 1
+        for (int qx = 0; qx < Q; ++qx) {
+            double nx0[P], nx1[P];
+#pragma unroll
+            for (int a = 0; a < P; ++a) {
+                nx0[a] = Nx[(qx * P + a) * 2];
+                nx1[a] = Nx[(qx * P + a) * 2 + 1];
+            }
+            // x stage of FormUn: u and du/dx along x at qx, for every (ay, az)
+            double ux0[P][P], ux1[P][P];
+            if constexpr (!TZERO) {
+#pragma unroll
+                for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+                    for (int az = 0; az < P; ++az) {
+                        const double* col = ce + static_cast<long>(sx) * ay + plane * az;
+                        double a0 = nx0[0] * col[0], a1 = nx1[0] * col[0];
+#pragma unroll
+                        for (int b = 1; b < P; ++b) {
+                            a0 = fma(nx0[b], col[b], a0);
+                            a1 = fma(nx1[b], col[b], a1);
+                        }
+                        ux0[ay][az] = a0;
+                        ux1[ay][az] = a1;
+                    }
+            }
+            double t0[P][P], tx[P][P];
+#pragma unroll
+            for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+                for (int az = 0; az < P; ++az) t0[ay][az] = tx[ay][az] = 0.0;
+            const double xq = ptsx[static_cast<size_t>(exc) * Q + qx];
+            const double wxq = Wx[qx];
+#pragma unroll 1
+            for (int qy = 0; qy < Q; ++qy) {
+                double ny0[P], ny1[P];
+#pragma unroll
+                for (int a = 0; a < P; ++a) {
+                    ny0[a] = Ny[(qy * P + a) * 2];
+                    ny1[a] = Ny[(qy * P + a) * 2 + 1];
+                }
+                double u00[P], u10[P], u01[P];  // y stage: values, d/dx, d/dy along z
+                if constexpr (!TZERO) {
+#pragma unroll
+                    for (int az = 0; az < P; ++az) {
+                        double a = ny0[0] * ux0[0][az], b = ny0[0] * ux1[0][az], g = ny1[0] * ux0[0][az];
+#pragma unroll
+                        for (int ay = 1; ay < P; ++ay) {
+                            a = fma(ny0[ay], ux0[ay][az], a);
+                            b = fma(ny0[ay], ux1[ay][az], b);
+                            g = fma(ny1[ay], ux0[ay][az], g);
+                        }
+                        u00[az] = a;
+                        u10[az] = b;
+                        u01[az] = g;
+                    }
+                }
+                double h0[P], hx[P], hy[P];
+#pragma unroll
+                for (int az = 0; az < P; ++az) h0[az] = hx[az] = hy[az] = 0.0;
+                const double yq = ptsy[static_cast<size_t>(ey) * Q + qy];
+                const double wxy = wxq * Wy[qy];
+#pragma unroll
+                for (int qz = 0; qz < Q; ++qz) {
+                    double nz0[P], nz1[P];
+#pragma unroll
+                    for (int a = 0; a < P; ++a) {
+                        nz0[a] = Nz[(qz * P + a) * 2];
+                        nz1[a] = Nz[(qz * P + a) * 2 + 1];
+                    }
+                    const double w = J * (wxy * Wz[qz]);
+                    const double zq = ptsz[static_cast<size_t>(ez) * Q + qz];
+                    double w0, gx = 0.0, gy = 0.0, gz = 0.0;
+                    if constexpr (TZERO) {
+                        const double f0 = oil_initial(o, xq, yq, zq);
+                        w0 = w * f0;
+                        l2 = fma(w0, f0, l2);
+                    } else {
+                        double u = nz0[0] * u00[0], dux = nz0[0] * u10[0], duy = nz0[0] * u01[0],
+                               duz = nz1[0] * u00[0];
+#pragma unroll
+                        for (int az = 1; az < P; ++az) {
+                            u = fma(nz0[az], u00[az], u);
+                            dux = fma(nz0[az], u10[az], dux);
+                            duy = fma(nz0[az], u01[az], duy);
+                            duz = fma(nz1[az], u00[az], duz);
+                        }
+                        const double kqv = kq[qx + Q * (qy + Q * (qz + static_cast<long>(Q) * e))];
+                        const double pump = oil_sources(o.pumps, o.npumps, o.pump_strength, o.radius, xq, yq, zq);
+                        const double dw = oil_sources(o.drains, o.ndrains, o.drain_strength, o.radius, xq, yq, zq);
+                        const double vd = fmax(0.0, dw * u);
+                        w0 = w * (u + dt * (pump - vd));
+                        const double g = -w * dt * kqv * exp(o.mi * u);
+                        gx = g * dux;
+                        gy = g * duy;
+                        gz = g * duz;
+                        l2 = fma(w * u, u, l2);
+                        dr = fma(w * dt, vd, dr);
+                    }
+                    // z stage of the transposed contraction
+#pragma unroll
+                    for (int az = 0; az < P; ++az) {
+                        h0[az] = fma(nz1[az], gz, fma(nz0[az], w0, h0[az]));
+                        hx[az] = fma(nz0[az], gx, hx[az]);
+                        hy[az] = fma(nz0[az], gy, hy[az]);
+                    }
+                }
+#pragma unroll
+                for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+                    for (int az = 0; az < P; ++az) {
+                        t0[ay][az] = fma(ny1[ay], hy[az], fma(ny0[ay], h0[az], t0[ay][az]));
+                        tx[ay][az] = fma(ny0[ay], hx[az], tx[ay][az]);
+                    }
+            }
+            if (active) {
+#pragma unroll
+                for (int ax = 0; ax < P; ++ax)
+#pragma unroll
+                    for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+                        for (int az = 0; az < P; ++az)
+                            atomic_add(&fs[slot[az]][ay][lane + ax], fma(nx1[ax], tx[ay][az], nx0[ax] * t0[ay][az]));
+            }
+        }
+        __syncthreads();
+        flush_plane(ez);
+        __syncthreads();
+    }
+    for (int z = ez1; z < ez1 + KL; ++z) flush_plane(z);
+
+    double l2d = wave_sum(active ? l2 : 0.0);
+    double drd = wave_sum(active ? dr : 0.0);
+    if (lane == 0) {
+        l2_part[blockIdx.x] = l2d;
+        if (dr_part) dr_part[blockIdx.x] = drd;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ part, int n,
                                                          double* __restrict__ out)
 {
@@ -525,6 +817,28 @@ hipError_t assemble_p(int form, int f0, const double* c, T* F, const Tables<T>& 
     return form == kFormHeat
                ? heat_p<T, P, kHeatPlain>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, nullptr, nullptr, s)
                : project_p<T, P>(F, tb, f0, l2_part, part_cap, nparts, s);
+}
+
+template <int P>
+hipError_t oil_p(bool tzero, const double* c, double* F, const Tables<double>& tb, const OilDev& o, const double* kq,
+                 double dt, double* l2_part, double* dr_part, int part_cap, int* nparts, hipStream_t s)
+{
+    AsmDims dm = base_dims(tb);
+    dm.yblocks = dm.ney;
+    dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.ney, dm.nez, 2048);
+    const int nseg = (dm.nez + dm.zseg - 1) / dm.zseg;
+    dim3 grid(dm.xchunks * dm.ney * nseg);
+    *nparts = static_cast<int>(grid.x);
+    if (*nparts > part_cap) return hipErrorInvalidValue;
+    if (tzero)
+        hipLaunchKernelGGL((k_oil<P, true>), grid, dim3(kWave), 0, s, c, F, tb.d[0].tab, tb.d[1].tab, tb.d[2].tab,
+                           tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts, tb.d[0].pts,
+                           tb.d[1].pts, tb.d[2].pts, dm, o, kq, dt, l2_part, dr_part);
+    else
+        hipLaunchKernelGGL((k_oil<P, false>), grid, dim3(kWave), 0, s, c, F, tb.d[0].tab, tb.d[1].tab, tb.d[2].tab,
+                           tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts, tb.d[0].pts,
+                           tb.d[1].pts, tb.d[2].pts, dm, o, kq, dt, l2_part, dr_part);
+    return hipGetLastError();
 }
 
 // Output path (SaveSplinePlot + EvalSpline, src/plot.F90:97-145,
@@ -632,5 +946,34 @@ template hipError_t launch_heat_adjoint<double>(int, const double*, const double
                                                 double, const double*, double*, double*, int, int*, hipStream_t);
 template hipError_t launch_heat_adjoint<float>(int, const double*, const double*, float*, const Tables<float>&,
                                                double, const double*, double*, double*, int, int*, hipStream_t);
+
+hipError_t launch_oil_kq(int p, const Tables<double>& tb, const OilDev& o, double* kq, hipStream_t s)
+{
+    const int nex = tb.d[0].nelem, ney = tb.d[1].nelem, nez = tb.d[2].nelem;
+    const long q3 = static_cast<long>(p + 1) * (p + 1) * (p + 1);
+    const long n = static_cast<long>(nex) * ney * nez * q3;
+    const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+    switch (p) {
+    case 1: hipLaunchKernelGGL((k_oil_kq<2>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 2: hipLaunchKernelGGL((k_oil_kq<3>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 3: hipLaunchKernelGGL((k_oil_kq<4>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 4: hipLaunchKernelGGL((k_oil_kq<5>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_oil(int p, bool tzero, const double* c, double* F, const Tables<double>& tb, const OilDev& o,
+                      const double* kq, double dt, double* l2_part, double* dr_part, int part_cap, int* nparts,
+                      hipStream_t s)
+{
+    switch (p) {
+    case 1: return oil_p<2>(tzero, c, F, tb, o, kq, dt, l2_part, dr_part, part_cap, nparts, s);
+    case 2: return oil_p<3>(tzero, c, F, tb, o, kq, dt, l2_part, dr_part, part_cap, nparts, s);
+    case 3: return oil_p<4>(tzero, c, F, tb, o, kq, dt, l2_part, dr_part, part_cap, nparts, s);
+    case 4: return oil_p<5>(tzero, c, F, tb, o, kq, dt, l2_part, dr_part, part_cap, nparts, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 }  // namespace ads

@@ -77,6 +77,21 @@ struct ads_ctx {
 
     std::vector<MixSet> mixsets;  // MultiStep operators factored so far
 
+    // the oil problem (ads_oil_setup)
+    struct Oil {
+        double* curves = nullptr;
+        long curves_cap = 0;
+        double* pumps = nullptr;
+        long pumps_cap = 0;
+        double* drains = nullptr;
+        long drains_cap = 0;
+        double* kq = nullptr;       // kq at every Gauss point
+        double* dr_part = nullptr;  // per-workgroup drained partial sums
+        double* drained = nullptr;  // device scalar: the last step's drained
+        ads::OilDev dev{};
+        bool ready = false;
+    } oil;
+
     std::vector<void*> allocs;
 };
 
@@ -341,6 +356,35 @@ int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, const double* 
     }
     if (x != c_out)
         ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    return ADS_OK;
+}
+
+// device copy of 3 n values (three arrays of n, or n triples when b and c
+// are null), reallocated when larger than before
+int oil_upload(ads_ctx* ctx, double** dst, long* cap, const double* a, const double* b, const double* c, long n)
+{
+    int rc;
+    if (*cap < 3 * n || !*dst) {
+        if ((rc = dev_alloc(ctx, dst, static_cast<size_t>(3 * n > 3 ? 3 * n : 3)))) return rc;
+        *cap = 3 * n;
+    }
+    if (n == 0) return ADS_OK;
+    if (b) {
+        ADS_HIP(ctx, hipMemcpyAsync(*dst, a, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        ADS_HIP(ctx, hipMemcpyAsync(*dst + n, b, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        ADS_HIP(ctx, hipMemcpyAsync(*dst + 2 * n, c, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        ADS_HIP(ctx, hipMemcpyAsync(*dst, a, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    return ADS_OK;
+}
+
+int read_scalar(ads_ctx* ctx, const double* d, double* h, const char* what)
+{
+    if (!h) return ADS_OK;
+    ADS_HIP(ctx, hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!std::isfinite(*h)) return fail(ctx, ADS_ERR_NUMERIC, std::string(what) + " is not finite");
     return ADS_OK;
 }
 
@@ -785,6 +829,86 @@ int ads_multistep(ads_ctx* ctx, const double mix[12], double dt, double* l2norm)
     }
     std::swap(ctx->c, ctx->F);
     return read_l2(ctx, l2norm);
+}
+
+int ads_oil_setup(ads_ctx* ctx, const double* cx, const double* cy, const double* cz, int cN, int cL,
+                  const double* pumps, int npumps, const double* drains, int ndrains, const double params[6])
+{
+    if (check_ctx(ctx) || !cx || !cy || !cz || !params) return ADS_ERR_INVALID;
+    if (cN < 1 || cL < 2 || npumps < 0 || ndrains < 0 || (npumps > 0 && !pumps) || (ndrains > 0 && !drains))
+        return fail(ctx, ADS_ERR_INVALID, "ads_oil_setup: need cN >= 1 curves of cL >= 2 points and valid sources");
+    for (int i = 0; i < 6; ++i)
+        if (!std::isfinite(params[i])) return fail(ctx, ADS_ERR_INVALID, "ads_oil_setup: parameters must be finite");
+    if (params[3] <= 0.0) return fail(ctx, ADS_ERR_INVALID, "ads_oil_setup: radius must be positive");
+    auto& o = ctx->oil;
+    o.ready = false;
+    const long npt = static_cast<long>(cN) * cL;
+    int rc;
+    if ((rc = oil_upload(ctx, &o.curves, &o.curves_cap, cx, cy, cz, npt))) return rc;
+    if ((rc = oil_upload(ctx, &o.pumps, &o.pumps_cap, pumps, nullptr, nullptr, npumps))) return rc;
+    if ((rc = oil_upload(ctx, &o.drains, &o.drains_cap, drains, nullptr, nullptr, ndrains))) return rc;
+    const long q3 = static_cast<long>(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1);
+    if (!o.kq && (rc = dev_alloc(ctx, &o.kq, static_cast<size_t>(ctx->E * q3)))) return rc;
+    if (!o.dr_part && (rc = dev_alloc(ctx, &o.dr_part, static_cast<size_t>(ctx->l2_cap)))) return rc;
+    if (!o.drained && (rc = dev_alloc(ctx, &o.drained, 1))) return rc;
+    o.dev = ads::OilDev{o.curves, cN, cL, o.pumps, npumps, o.drains, ndrains,
+                        params[0], params[1], params[2], params[3], params[4], params[5]};
+    ADS_HIP(ctx, ads::launch_oil_kq(ctx->p, ctx->tb64, o.dev, o.kq, ctx->stream));
+    // the host arrays may be released once this returns
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    o.ready = true;
+    return ADS_OK;
+}
+
+int ads_oil_project(ads_ctx* ctx, double* l2norm)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    if (!ctx->oil.ready) return fail(ctx, ADS_ERR_INVALID, "ads_oil_project: call ads_oil_setup first");
+    int rc = zero(ctx, ctx->F, ctx->D);
+    if (rc) return rc;
+    int nparts = 0;
+    ADS_HIP(ctx, ads::launch_oil(ctx->p, true, nullptr, ctx->F, ctx->tb64, ctx->oil.dev, ctx->oil.kq, 0.0,
+                                 ctx->l2_part, nullptr, ctx->l2_cap, &nparts, ctx->stream));
+    ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, ctx->l2, ctx->stream));
+    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, ctx->sw64, false, ctx->stream));
+    std::swap(ctx->c, ctx->F);
+    return read_l2(ctx, l2norm);
+}
+
+int ads_oil_step(ads_ctx* ctx, double dt, double* l2norm, double* drained)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    if (!ctx->oil.ready) return fail(ctx, ADS_ERR_INVALID, "ads_oil_step: call ads_oil_setup first");
+    if (!std::isfinite(dt)) return fail(ctx, ADS_ERR_INVALID, "dt is not finite");
+    int rc = zero(ctx, ctx->F, ctx->D);
+    if (rc) return rc;
+    int nparts = 0;
+    ADS_HIP(ctx, ads::launch_oil(ctx->p, false, ctx->c, ctx->F, ctx->tb64, ctx->oil.dev, ctx->oil.kq, dt,
+                                 ctx->l2_part, ctx->oil.dr_part, ctx->l2_cap, &nparts, ctx->stream));
+    ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, ctx->l2, ctx->stream));
+    ADS_HIP(ctx, ads::launch_reduce_partials(ctx->oil.dr_part, nparts, ctx->oil.drained, ctx->stream));
+    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, ctx->sw64, false, ctx->stream));
+    std::swap(ctx->c, ctx->F);
+    if ((rc = read_l2(ctx, l2norm))) return rc;
+    return read_scalar(ctx, ctx->oil.drained, drained, "drained");
+}
+
+int ads_oil_get_kq(ads_ctx* ctx, double* kq_host)
+{
+    if (check_ctx(ctx) || !kq_host) return ADS_ERR_INVALID;
+    if (!ctx->oil.ready) return fail(ctx, ADS_ERR_INVALID, "ads_oil_get_kq: call ads_oil_setup first");
+    const long q3 = static_cast<long>(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1);
+    ADS_HIP(ctx, hipMemcpyAsync(kq_host, ctx->oil.kq, sizeof(double) * ctx->E * q3, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ADS_OK;
+}
+
+int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz)
+{
+    if (cN < 1 || cL < 2 || !cx || !cy || !cz) return ADS_ERR_INVALID;
+    ads::oil_curves(seed, cN, cL, cx, cy, cz);
+    return ADS_OK;
 }
 
 int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms_per_launch)

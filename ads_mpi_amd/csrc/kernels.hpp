@@ -93,6 +93,26 @@ struct BandSet {
 template <typename TX, typename TR>
 hipError_t launch_kron_residual(int p, const double* c, const TX* x, const double* b, TR* r, const BandSet& bs,
                                 double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s);
+// The oil problem's inputs on the device (problems/oil/input_data.F90):
+// curve points (cx, cy, cz of cN curves x cL points, consecutive), pumps and
+// drains (x, y, z each) and the scalar constants.
+struct OilDev {
+    const double* curves;
+    int cN, cL;
+    const double* pumps;
+    int npumps;
+    const double* drains;
+    int ndrains;
+    double mi, kq_min, kq_max, radius, pump_strength, drain_strength;
+};
+// kq at every Gauss point, [kx + Q (ky + Q (kz + Q e))] (CacheKqValues)
+hipError_t launch_oil_kq(int p, const Tables<double>& tb, const OilDev& o, double* kq, hipStream_t s);
+// the oil right-hand side: t > 0 from c (tzero false) or the projection of
+// the initial state; per-workgroup partial sums of l2 and drained
+hipError_t launch_oil(int p, bool tzero, const double* c, double* F, const Tables<double>& tb, const OilDev& o,
+                      const double* kq, double dt, double* l2_part, double* dr_part, int part_cap, int* nparts,
+                      hipStream_t s);
+
 // upper bound of the workgroup (partial-sum) count of launch_kron_residual
 long kron_residual_partials_bound(const BandSet& bs);
 

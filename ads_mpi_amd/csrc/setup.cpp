@@ -380,6 +380,32 @@ static inline double unit_from(uint64_t key)
     return static_cast<double>(splitmix64(key) >> 11) * 0x1.0p-53;
 }
 
+void oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz)
+{
+    const uint64_t base = seed * 0xD1B54A32D192ED03ULL + 0x2545F4914F6CDD1DULL;
+    uint64_t k = 0;
+    const double step = 0.05;
+    for (int i = 0; i < cN; ++i) {
+        double x[3], dx[3];
+        for (int d = 0; d < 3; ++d) x[d] = unit_from(base + k++);
+        cx[i * cL] = x[0];
+        cy[i * cL] = x[1];
+        cz[i * cL] = x[2];
+        for (int d = 0; d < 3; ++d) dx[d] = unit_from(base + k++);
+        for (int j = 1; j < cL; ++j) {
+            double ddx[3];
+            k += 3;  // random_number(t) is drawn and not used (input_data.F90:153)
+            for (int d = 0; d < 3; ++d) ddx[d] = unit_from(base + k++);
+            const double ss = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            for (int d = 0; d < 3; ++d) ddx[d] = 2.0 * (ddx[d] - 0.5) - 0.4 * dx[d] * ss;
+            for (int d = 0; d < 3; ++d) dx[d] = dx[d] + 0.4 * ddx[d];
+            cx[i * cL + j] = cx[i * cL + j - 1] + step * dx[0];
+            cy[i * cL + j] = cy[i * cL + j - 1] + step * dx[1];
+            cz[i * cL + j] = cz[i * cL + j - 1] + step * dx[2];
+        }
+    }
+}
+
 void fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out)
 {
     const uint64_t base = seed * 0xD1B54A32D192ED03ULL;

@@ -87,6 +87,9 @@ void grid_tables(const Direction& d, int res, double lo, double hi, std::vector<
 // Synthetic input generator shared with the host API (SplitMix64 counters).
 uint64_t splitmix64(uint64_t x);
 void fill_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
+// oil curves: the reference's random walk (problems/oil/input_data.F90:139-165)
+// with uniforms from SplitMix64(seed) in the same call order
+void oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz);
 void fill_lognormal(uint64_t seed, int64_t n, double mu, double sigma, double* out);
 
 }  // namespace ads

@@ -171,6 +171,38 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
                        const double* kappa_d, double dt, int prec, int refine, double* lam_out_d,
                        double* dkappa_d);
 
+/* ---- the oil problem (problems/oil) ---------------------------------------
+ * Integrand (problems/oil/RHS_eq.F90:76-91) at t > 0:
+ *   J W (v u + Dt (-kq(X) exp(mi u) grad u . grad v + v (pump(X) - max(0, drain(X) u))))
+ * with kq(X) = lerp(falloff(0, 0.06, dist(X)), Kqmin, Kqmax), dist the
+ * distance from the curves, pump/drain sums of strength * falloff(0, radius,
+ * |source - X|) (input_data.F90:171-303); at t = 0 the projection of
+ * initial_state (:317-329).  The reference draws the curves from an unseeded
+ * random_number (InitInputData, :139-165) and reads pumps and drains from the
+ * command line (InitPumps, :99-136); here all of them are inputs. */
+
+/* InitInputData's curves, PrecomputeKq (input_data.F90:384-395) and the
+ * problem constants.  cx, cy, cz: cN curves of cL points (point j of curve i
+ * at [i*cL + j]); pumps, drains: x, y, z triples; params = {mi, Kqmin, Kqmax,
+ * radius, pumping_strength, draining_strength} (input_data.F90:6-13).  Caches
+ * kq at every Gauss point on the device.  Host arrays may be freed on return. */
+int ads_oil_setup(ads_ctx* ctx, const double* cx, const double* cy, const double* cz, int cN, int cL,
+                  const double* pumps, int npumps, const double* drains, int ndrains,
+                  const double params[6]);
+/* Step(0, ...) of problems/oil/main.F90:66-77: c = M^-1 F(initial_state). */
+int ads_oil_project(ads_ctx* ctx, double* l2norm);
+/* Step(iter > 0, ...) with the oil integrand: c <- M^-1 F(c).  l2norm and
+ * drained (either may be NULL: no host sync) receive sum J W v u^2 and this
+ * step's sum J W v Dt vdrain (the reference's module variable `drained`
+ * accumulates these over all steps, RHS_eq.F90:86). */
+int ads_oil_step(ads_ctx* ctx, double dt, double* l2norm, double* drained);
+/* the cached kq values, Kqvals(kx, ky, kz, ex, ey, ez) order (kx fastest):
+ * E (p+1)^3 doubles */
+int ads_oil_get_kq(ads_ctx* ctx, double* kq_host);
+/* curves by InitInputData's random walk with uniforms from SplitMix64(seed)
+ * (the reference's own stream is unseeded): cN*cL points each of cx, cy, cz */
+int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz);
+
 /* ---- output ---------------------------------------------------------------- */
 
 /* The spline with the context's coefficients on a tensor grid, for

@@ -988,6 +988,245 @@ int orc_multistep(const orc_setup* s, const double mix[12], const double* c_in, 
     return rc;
 }
 
+/* ======================================================================
+ * the oil problem -- problems/oil/RHS_eq.F90:69-91, input_data.F90:139-380,
+ * src/math.F90:44-88
+ * ====================================================================== */
+
+double orc_oil_lerp(double t, double x, double y) { return (1.0 - t) * x + t * y; }
+
+double orc_oil_falloff(double r, double R, double t)
+{
+    if (t < r) return 1.0;
+    if (t > R) return 0.0;
+    double h = (t - r) / (R - r);
+    double g = (h - 1.0) * (h + 1.0);
+    return g * g;
+}
+
+double orc_oil_dist_from_segment(const double p[3], const double a[3], const double b[3])
+{
+    double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    double cx = p[0] - a[0], cy = p[1] - a[1], cz = p[2] - a[2];
+    double dot = dx * cx + dy * cy + dz * cz;
+    double len2 = dx * dx + dy * dy + dz * dz;
+    double proj = dot / len2;
+    double xx, yy, zz;
+    if (proj < 0) { xx = a[0]; yy = a[1]; zz = a[2]; }
+    else if (proj > 1) { xx = b[0]; yy = b[1]; zz = b[2]; }
+    else { xx = a[0] + proj * dx; yy = a[1] + proj * dy; zz = a[2] + proj * dz; }
+    double ex = p[0] - xx, ey = p[1] - yy, ez = p[2] - zz;
+    return ex * ex + ey * ey + ez * ez;
+}
+
+double orc_oil_dist_from_curves(const orc_oil* o, const double p[3])
+{
+    double fval = 1e3;
+    for (int i = 0; i < o->cN; ++i)
+        for (int j = 1; j < o->cL; ++j) {
+            int k = i * o->cL + j;
+            double a[3] = {o->cx[k - 1], o->cy[k - 1], o->cz[k - 1]};
+            double b[3] = {o->cx[k], o->cy[k], o->cz[k]};
+            double d = orc_oil_dist_from_segment(p, a, b);
+            if (d < fval) fval = d;
+        }
+    return fval;
+}
+
+double orc_oil_kq(const orc_oil* o, double x, double y, double z)
+{
+    double p[3] = {x, y, z};
+    double dist = sqrt(orc_oil_dist_from_curves(o, p));
+    return orc_oil_lerp(orc_oil_falloff(0.0, 0.06, dist), o->kq_min, o->kq_max);
+}
+
+static double oil_norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+double orc_oil_pumping(const orc_oil* o, double x, double y, double z)
+{
+    double fval = 0.0;
+    for (int i = 0; i < o->npumps; ++i) {
+        const double* q = o->pumps + 3 * i;
+        fval = fval + o->pump_strength * orc_oil_falloff(0.0, o->radius, oil_norm3(q[0] - x, q[1] - y, q[2] - z));
+    }
+    return fval;
+}
+
+double orc_oil_draining(const orc_oil* o, double u, double x, double y, double z)
+{
+    double fval = 0.0;
+    for (int i = 0; i < o->ndrains; ++i) {
+        const double* q = o->drains + 3 * i;
+        fval = fval + o->drain_strength * orc_oil_falloff(0.0, o->radius, oil_norm3(q[0] - x, q[1] - y, q[2] - z));
+    }
+    return fval * u;
+}
+
+double orc_oil_initial_state(const orc_oil* o, double x, double y, double z)
+{
+    double p[3] = {x, y, z};
+    double dist = sqrt(orc_oil_dist_from_curves(o, p));
+    /* bump3d(0.2, 0.6, x, y, z) = falloff(0.1, 0.3, |X - 0.5|) (math.F90:173-181) */
+    double b = orc_oil_falloff(0.2 / 2.0, 0.6 / 2.0, oil_norm3(x - 0.5, y - 0.5, z - 0.5));
+    return 0.1 * orc_oil_lerp(orc_oil_falloff(0.0, 0.1, dist), 0.0, 1.0) * b;
+}
+
+int orc_oil_kq_cache(const orc_setup* s, const orc_oil* o, double* kq)
+{
+    const int qx = s->q[0], qy = s->q[1], qz = s->q[2];
+    const int nex = s->nelem[0], ney = s->nelem[1], nez = s->nelem[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int ez = 0; ez < nez; ++ez)
+        for (int ey = 0; ey < ney; ++ey)
+            for (int ex = 0; ex < nex; ++ex)
+                for (int kz = 0; kz < qz; ++kz)
+                    for (int ky = 0; ky < qy; ++ky)
+                        for (int kx = 0; kx < qx; ++kx)
+                            kq[kx + qx * (ky + qy * (kz + (size_t)qz * (ex + (size_t)nex * (ey + (size_t)ney * ez))))] =
+                                orc_oil_kq(o, s->X[0][kx + qx * ex], s->X[1][ky + qy * ey], s->X[2][kz + qz * ez]);
+    return 0;
+}
+
+/* FormUn + the per (point, basis function) integrand loop for one oil
+ * element (as element_faithful); returns l2, adds the element's drained */
+static double element_oil(const orc_setup* s, const orc_oil* o, const double* kqv, const double* c, double dt,
+                          int tzero, int ex, int ey, int ez, double* elarr, double* drained)
+{
+    const int px = s->p[0], py = s->p[1], pz = s->p[2];
+    const int qx = s->q[0], qy = s->q[1], qz = s->q[2];
+    const int sx = s->dofs[0], sy = s->dofs[1];
+    const int nex = s->nelem[0], ney = s->nelem[1];
+    double un[11 * 11 * 11], dun[3][11 * 11 * 11];
+    double l2 = 0.0, dr = 0.0;
+    const int ox = s->O[0][ex], oy = s->O[1][ey], oz = s->O[2][ez];
+    double J = s->J[0][ex] * s->J[1][ey] * s->J[2][ez];
+    if (!tzero) {
+        for (int kx = 0; kx < qx; ++kx)
+            for (int ky = 0; ky < qy; ++ky)
+                for (int kz = 0; kz < qz; ++kz) {
+                    double Uval = 0.0, dux = 0.0, duy = 0.0, duz = 0.0;
+                    for (int bx = 0; bx <= px; ++bx)
+                        for (int by = 0; by <= py; ++by)
+                            for (int bz = 0; bz <= pz; ++bz) {
+                                double Uc = c[(size_t)(ox + bx) + (size_t)sx * ((oy + by) + (size_t)sy * (oz + bz))];
+                                double v = NNA(s, 0, 0, bx, kx, ex) * NNA(s, 1, 0, by, ky, ey) * NNA(s, 2, 0, bz, kz, ez);
+                                double dvx = NNA(s, 0, 1, bx, kx, ex) * NNA(s, 1, 0, by, ky, ey) * NNA(s, 2, 0, bz, kz, ez);
+                                double dvy = NNA(s, 0, 0, bx, kx, ex) * NNA(s, 1, 1, by, ky, ey) * NNA(s, 2, 0, bz, kz, ez);
+                                double dvz = NNA(s, 0, 0, bx, kx, ex) * NNA(s, 1, 0, by, ky, ey) * NNA(s, 2, 1, bz, kz, ez);
+                                Uval = Uval + Uc * v;
+                                dux = dux + Uc * dvx;
+                                duy = duy + Uc * dvy;
+                                duz = duz + Uc * dvz;
+                            }
+                    int kk = kx + qx * (ky + qy * kz);
+                    un[kk] = Uval; dun[0][kk] = dux; dun[1][kk] = duy; dun[2][kk] = duz;
+                }
+    }
+    for (int i = 0; i < (px + 1) * (py + 1) * (pz + 1); ++i) elarr[i] = 0.0;
+    for (int kx = 0; kx < qx; ++kx)
+        for (int ky = 0; ky < qy; ++ky)
+            for (int kz = 0; kz < qz; ++kz) {
+                double W = s->W[0][kx] * s->W[1][ky] * s->W[2][kz];
+                int kk = kx + qx * (ky + qy * kz);
+                double X = s->X[0][kx + qx * ex], Y = s->X[1][ky + qy * ey], Z = s->X[2][kz + qz * ez];
+                double vpump = 0.0, vdrain = 0.0, kqval = 0.0, f0 = 0.0;
+                if (tzero) {
+                    f0 = orc_oil_initial_state(o, X, Y, Z);
+                } else {
+                    vpump = orc_oil_pumping(o, X, Y, Z);
+                    kqval = kqv[kx + qx * (ky + qy * (kz + (size_t)qz * (ex + (size_t)nex * (ey + (size_t)ney * ez))))];
+                    double d = orc_oil_draining(o, un[kk], X, Y, Z);
+                    vdrain = d > 0.0 ? d : 0.0;
+                }
+                double fval = vpump - vdrain;
+                for (int ax = 0; ax <= px; ++ax)
+                    for (int ay = 0; ay <= py; ++ay)
+                        for (int az = 0; az <= pz; ++az) {
+                            double v = NNA(s, 0, 0, ax, kx, ex) * NNA(s, 1, 0, ay, ky, ey) * NNA(s, 2, 0, az, kz, ez);
+                            double ret;
+                            if (!tzero) {
+                                double dvx = NNA(s, 0, 1, ax, kx, ex) * NNA(s, 1, 0, ay, ky, ey) * NNA(s, 2, 0, az, kz, ez);
+                                double dvy = NNA(s, 0, 0, ax, kx, ex) * NNA(s, 1, 1, ay, ky, ey) * NNA(s, 2, 0, az, kz, ez);
+                                double dvz = NNA(s, 0, 0, ax, kx, ex) * NNA(s, 1, 0, ay, ky, ey) * NNA(s, 2, 1, az, kz, ez);
+                                double rhs = dt * (-kqval * exp(o->mi * un[kk]) *
+                                                       (dun[0][kk] * dvx + dun[1][kk] * dvy + dun[2][kk] * dvz) +
+                                                   v * fval);
+                                ret = J * W * (v * un[kk] + rhs);
+                                dr = dr + J * W * v * dt * vdrain;
+                                l2 += J * W * v * un[kk] * un[kk];
+                            } else {
+                                ret = J * W * v * f0;
+                                l2 += J * W * v * f0 * f0;
+                            }
+                            elarr[ax + (px + 1) * (ay + (py + 1) * az)] += ret;
+                        }
+            }
+    *drained += dr;
+    return l2;
+}
+
+/* the oil right-hand side over all elements, slab-parallel as form_rhs */
+static int oil_form_rhs(const orc_setup* s, const orc_oil* o, const double* kq, const double* c, double dt, int tzero,
+                        double* F, double* l2norm, double* drained)
+{
+    const int nex = s->nelem[0], ney = s->nelem[1], nez = s->nelem[2];
+    const int px = s->p[0], py = s->p[1], pz = s->p[2];
+    const int sx = s->dofs[0], sy = s->dofs[1], sz = s->dofs[2];
+    if (px > 10 || py > 10 || pz > 10) return -1;
+    memset(F, 0, sizeof(double) * (size_t)sx * sy * sz);
+    const int T = px > 4 ? px : 4;
+    const int nslab = (nex + T - 1) / T;
+    double* slab = (double*)calloc((size_t)nslab * 2, sizeof(double));
+    for (int phase = 0; phase < 2; ++phase) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int sl = phase; sl < nslab; sl += 2) {
+            double elarr[11 * 11 * 11];
+            double l2 = 0.0, dr = 0.0;
+            int e0 = sl * T, e1 = (sl + 1) * T < nex ? (sl + 1) * T : nex;
+            for (int ex = e0; ex < e1; ++ex)
+                for (int ey = 0; ey < ney; ++ey)
+                    for (int ez = 0; ez < nez; ++ez) {
+                        l2 += element_oil(s, o, kq, c, dt, tzero, ex, ey, ez, elarr, &dr);
+                        const int ox = s->O[0][ex], oy = s->O[1][ey], oz = s->O[2][ez];
+                        for (int ax = 0; ax <= px; ++ax)
+                            for (int ay = 0; ay <= py; ++ay)
+                                for (int az = 0; az <= pz; ++az)
+                                    F[(size_t)(ox + ax) + (size_t)sx * ((oy + ay) + (size_t)sy * (oz + az))] +=
+                                        elarr[ax + (px + 1) * (ay + (py + 1) * az)];
+                    }
+            slab[2 * sl] = l2;
+            slab[2 * sl + 1] = dr;
+        }
+    }
+    double t2 = 0.0, td = 0.0;
+    for (int sl = 0; sl < nslab; ++sl) { t2 += slab[2 * sl]; td += slab[2 * sl + 1]; }
+    free(slab);
+    if (l2norm) *l2norm = t2;
+    if (drained) *drained = td;
+    return 0;
+}
+
+int orc_oil_form_rhs(const orc_setup* s, const orc_oil* o, const double* kq, const double* c, double dt, int tzero,
+                     double* F, double* l2norm, double* drained)
+{
+    return oil_form_rhs(s, o, kq, c, dt, tzero, F, l2norm, drained);
+}
+
+int orc_oil_step(const orc_setup* s, const orc_oil* o, const double* kq, const double* c_in, double dt,
+                 double* c_out, double* l2norm, double* drained)
+{
+    int rc = oil_form_rhs(s, o, kq, c_in, dt, 0, c_out, l2norm, drained);
+    if (rc) return rc;
+    return orc_solve_kron(s, 1, c_out, 0);
+}
+
+int orc_oil_project(const orc_setup* s, const orc_oil* o, double* c_out, double* l2norm)
+{
+    int rc = oil_form_rhs(s, o, NULL, NULL, 0.0, 1, c_out, l2norm, NULL);
+    if (rc) return rc;
+    return orc_solve_kron(s, 1, c_out, 0);
+}
+
 int orc_project(const orc_setup* s, int mode, int f0_kind, double* c_out, double* l2norm)
 {
     int rc = orc_form_rhs_project(s, mode, f0_kind, c_out, l2norm);
@@ -1067,6 +1306,31 @@ uint64_t orc_splitmix64(uint64_t x)
 static double u01(uint64_t key)
 {
     return (double)(orc_splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void orc_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz)
+{
+    uint64_t base = seed * 0xD1B54A32D192ED03ULL + 0x2545F4914F6CDD1DULL, k = 0;
+    const double step = 0.05;
+    for (int i = 0; i < cN; ++i) {
+        double x[3], dx[3];
+        for (int d = 0; d < 3; ++d) x[d] = u01(base + k++);
+        cx[i * cL] = x[0];
+        cy[i * cL] = x[1];
+        cz[i * cL] = x[2];
+        for (int d = 0; d < 3; ++d) dx[d] = u01(base + k++);
+        for (int j = 1; j < cL; ++j) {
+            double ddx[3];
+            k += 3; /* random_number(t): drawn and unused (input_data.F90:153) */
+            for (int d = 0; d < 3; ++d) ddx[d] = u01(base + k++);
+            double ss = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            for (int d = 0; d < 3; ++d) ddx[d] = 2.0 * (ddx[d] - 0.5) - 0.4 * dx[d] * ss;
+            for (int d = 0; d < 3; ++d) dx[d] = dx[d] + 0.4 * ddx[d];
+            cx[i * cL + j] = cx[i * cL + j - 1] + step * dx[0];
+            cy[i * cL + j] = cy[i * cL + j - 1] + step * dx[1];
+            cz[i * cL + j] = cz[i * cL + j - 1] + step * dx[2];
+        }
+    }
 }
 
 void orc_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out)

@@ -90,6 +90,55 @@ int orc_project(const orc_setup* s, int mode, int f0_kind, double* c_out, double
 int orc_heat_step_adjoint(const orc_setup* s, const double* c_n, const double* kappa, double dt,
                           const double* lam_next, double* lam_out, double* dkappa);
 
+/* ---- the oil problem (problems/oil) ------------------------------------
+ * Integrand (problems/oil/RHS_eq.F90:69-91), t > 0:
+ *   ret = J W (v u + Dt (-kq(X) exp(mi u) grad u . grad v + v (pump(X) - max(0, drain(X) u))))
+ *   l2norm += J W v u^2,  drained += J W v Dt max(0, drain(X) u)
+ * t = 0: ret = J W v u0(X) with u0 = initial_state (input_data.F90:317-329).
+ * drained is the step's share of the reference's running total (RHS_eq.F90:86;
+ * module variable, never reset, printed by ComputeResults).
+ * Curves, pumps and drains are inputs (the reference draws the curves from
+ * an unseeded random_number and reads pumps/drains from the command line). */
+typedef struct {
+    const double* cx;  /* curve points, cN * cL each (input_data.F90:8-9) */
+    const double* cy;
+    const double* cz;
+    int cN, cL;
+    const double* pumps;  /* x, y, z per pump */
+    int npumps;
+    const double* drains; /* x, y, z per drain */
+    int ndrains;
+    double mi, kq_min, kq_max, radius, pump_strength, drain_strength;
+} orc_oil;
+
+/* math.F90:44-51, 74-88 */
+double orc_oil_lerp(double t, double x, double y);
+double orc_oil_falloff(double r, double R, double t);
+/* squared distance of p from segment [a, b] (input_data.F90:171-205) */
+double orc_oil_dist_from_segment(const double p[3], const double a[3], const double b[3]);
+/* min squared distance from the curves' segments, at most 1e3 (:212-238) */
+double orc_oil_dist_from_curves(const orc_oil* o, const double p[3]);
+double orc_oil_kq(const orc_oil* o, double x, double y, double z);          /* :290-303 */
+double orc_oil_pumping(const orc_oil* o, double x, double y, double z);     /* :244-262 */
+double orc_oil_draining(const orc_oil* o, double u, double x, double y, double z); /* :269-287 */
+double orc_oil_initial_state(const orc_oil* o, double x, double y, double z);      /* :317-329 */
+/* Kqvals(kx, ky, kz, ex, ey, ez) at the Gauss points (CacheKqValues,
+ * input_data.F90:333-380), kx fastest */
+int orc_oil_kq_cache(const orc_setup* s, const orc_oil* o, double* kq);
+/* the oil right-hand side (Form3DRHS with the oil integrand; tzero selects
+ * the t = 0 branch), faithful loops */
+int orc_oil_form_rhs(const orc_setup* s, const orc_oil* o, const double* kq, const double* c, double dt, int tzero,
+                     double* F, double* l2norm, double* drained);
+/* one oil step with the faithful loops (Step with problems/oil RHS at t > 0);
+ * drained (may be NULL) receives this step's sum J W v Dt vdrain */
+int orc_oil_step(const orc_setup* s, const orc_oil* o, const double* kq, const double* c_in, double dt,
+                 double* c_out, double* l2norm, double* drained);
+/* t = 0: c = M^-1 F(initial_state) */
+int orc_oil_project(const orc_setup* s, const orc_oil* o, double* c_out, double* l2norm);
+/* curves by the reference's random walk (InitInputData, input_data.F90:139-165)
+ * with uniforms from SplitMix64(seed) in the same call order */
+void orc_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz);
+
 /* ---- synthetic inputs ------------------------------------------------- */
 uint64_t orc_splitmix64(uint64_t x);
 void     orc_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);

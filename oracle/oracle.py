@@ -98,8 +98,86 @@ def lib():
         L.orc_lognormal.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_splitmix64.argtypes = [ctypes.c_uint64]
         L.orc_splitmix64.restype = ctypes.c_uint64
+        for nm, nargs in (("orc_oil_lerp", 3), ("orc_oil_falloff", 3)):
+            getattr(L, nm).argtypes = [ctypes.c_double] * nargs
+            getattr(L, nm).restype = ctypes.c_double
+        L.orc_oil_dist_from_segment.argtypes = [_dp, _dp, _dp]
+        L.orc_oil_dist_from_segment.restype = ctypes.c_double
+        L.orc_oil_dist_from_curves.argtypes = [ctypes.POINTER(_OilC), _dp]
+        L.orc_oil_dist_from_curves.restype = ctypes.c_double
+        for nm in ("orc_oil_kq", "orc_oil_pumping", "orc_oil_initial_state"):
+            getattr(L, nm).argtypes = [ctypes.POINTER(_OilC)] + [ctypes.c_double] * 3
+            getattr(L, nm).restype = ctypes.c_double
+        L.orc_oil_draining.argtypes = [ctypes.POINTER(_OilC)] + [ctypes.c_double] * 4
+        L.orc_oil_draining.restype = ctypes.c_double
+        L.orc_oil_kq_cache.argtypes = [ctypes.c_void_p, ctypes.POINTER(_OilC), _dp]
+        L.orc_oil_form_rhs.argtypes = [ctypes.c_void_p, ctypes.POINTER(_OilC), _dp, _dp, ctypes.c_double,
+                                       ctypes.c_int, _dp, _dp, _dp]
+        L.orc_oil_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(_OilC), _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_oil_project.argtypes = [ctypes.c_void_p, ctypes.POINTER(_OilC), _dp, _dp]
+        L.orc_oil_curves.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
+
+
+class _OilC(ctypes.Structure):
+    _fields_ = [("cx", _dp), ("cy", _dp), ("cz", _dp), ("cN", ctypes.c_int), ("cL", ctypes.c_int),
+                ("pumps", _dp), ("npumps", ctypes.c_int), ("drains", _dp), ("ndrains", ctypes.c_int),
+                ("mi", ctypes.c_double), ("kq_min", ctypes.c_double), ("kq_max", ctypes.c_double),
+                ("radius", ctypes.c_double), ("pump_strength", ctypes.c_double),
+                ("drain_strength", ctypes.c_double)]
+
+
+class Oil:
+    """Inputs of the oil problem (problems/oil/input_data.F90): curves
+    (cN x cL points each of cx, cy, cz), pumps and drains (rows x, y, z) and
+    the constants mi, Kqmin, Kqmax, radius, pumping/draining strength."""
+
+    def __init__(self, cx, cy, cz, cN, cL, pumps=(), drains=(), mi=10.0, kq_min=1.0, kq_max=1000.0,
+                 radius=0.15, pump_strength=1.0, drain_strength=1.0):
+        self.cx = np.ascontiguousarray(cx, dtype=np.float64)
+        self.cy = np.ascontiguousarray(cy, dtype=np.float64)
+        self.cz = np.ascontiguousarray(cz, dtype=np.float64)
+        self.pumps = np.ascontiguousarray(np.reshape(pumps, (-1, 3)), dtype=np.float64)
+        self.drains = np.ascontiguousarray(np.reshape(drains, (-1, 3)), dtype=np.float64)
+        self.c = _OilC(_d(self.cx), _d(self.cy), _d(self.cz), int(cN), int(cL), _d(self.pumps),
+                       len(self.pumps), _d(self.drains), len(self.drains), mi, kq_min, kq_max, radius,
+                       pump_strength, drain_strength)
+        self.cN, self.cL = int(cN), int(cL)
+
+    @classmethod
+    def random(cls, seed, cN=30, cL=16, **kw):
+        cx, cy, cz = (np.zeros(cN * cL) for _ in range(3))
+        lib().orc_oil_curves(seed, cN, cL, _d(cx), _d(cy), _d(cz))
+        return cls(cx, cy, cz, cN, cL, **kw)
+
+    def kq(self, x, y, z):
+        return lib().orc_oil_kq(ctypes.byref(self.c), x, y, z)
+
+    def pumping(self, x, y, z):
+        return lib().orc_oil_pumping(ctypes.byref(self.c), x, y, z)
+
+    def draining(self, u, x, y, z):
+        return lib().orc_oil_draining(ctypes.byref(self.c), u, x, y, z)
+
+    def initial_state(self, x, y, z):
+        return lib().orc_oil_initial_state(ctypes.byref(self.c), x, y, z)
+
+    def dist_from_curves(self, p):
+        return lib().orc_oil_dist_from_curves(ctypes.byref(self.c), _d(np.ascontiguousarray(p, dtype=np.float64)))
+
+
+def oil_falloff(r, R, t):
+    return lib().orc_oil_falloff(r, R, t)
+
+
+def oil_lerp(t, x, y):
+    return lib().orc_oil_lerp(t, x, y)
+
+
+def oil_dist_from_segment(p, a, b):
+    f = lambda v: _d(np.ascontiguousarray(v, dtype=np.float64))
+    return lib().orc_oil_dist_from_segment(f(p), f(a), f(b))
 
 
 def ref():
@@ -121,6 +199,9 @@ def ref():
         R.ref_eval_spline.argtypes = [ctypes.c_int] * 6 + [_dp, _dp, _dp, _dp] + [ctypes.c_double] * 3
         R.ref_eval_spline.restype = ctypes.c_double
         R.ref_reorder.argtypes = [ctypes.c_int, _ip, _dp, _dp]
+        for nm, nargs in (("ref_falloff", 3), ("ref_lerp", 3), ("ref_bump3d", 5)):
+            getattr(R, nm).argtypes = [ctypes.c_double] * nargs
+            getattr(R, nm).restype = ctypes.c_double
         _ref = R
     return _ref
 
@@ -363,6 +444,40 @@ class Setup:
         rc = lib().orc_project(self.h, mode, f0, _d(out), ctypes.byref(l2))
         if rc:
             raise RuntimeError("project rc=%d" % rc)
+        return out, l2.value
+
+    def oil_kq_cache(self, oil):
+        kq = np.zeros(self.E * int(np.prod([p + 1 for p in self.p])))
+        lib().orc_oil_kq_cache(self.h, ctypes.byref(oil.c), _d(kq))
+        return kq
+
+    def oil_form_rhs(self, oil, kq, c, dt, tzero=False):
+        F = self._vec()
+        l2 = ctypes.c_double(0.0)
+        dr = ctypes.c_double(0.0)
+        cc = np.ascontiguousarray(c, dtype=np.float64).ravel() if c is not None else None
+        lib().orc_oil_form_rhs(self.h, ctypes.byref(oil.c), _d(kq) if kq is not None else None,
+                               _d(cc) if cc is not None else None, dt, int(tzero), _d(F), ctypes.byref(l2),
+                               ctypes.byref(dr))
+        return F, l2.value, dr.value
+
+    def oil_step(self, oil, kq, c, dt):
+        c = np.ascontiguousarray(c, dtype=np.float64).ravel()
+        out = self._vec()
+        l2 = ctypes.c_double(0.0)
+        dr = ctypes.c_double(0.0)
+        rc = lib().orc_oil_step(self.h, ctypes.byref(oil.c), _d(kq), _d(c), dt, _d(out), ctypes.byref(l2),
+                                ctypes.byref(dr))
+        if rc:
+            raise RuntimeError("oil step rc=%d" % rc)
+        return out, l2.value, dr.value
+
+    def oil_project(self, oil):
+        out = self._vec()
+        l2 = ctypes.c_double(0.0)
+        rc = lib().orc_oil_project(self.h, ctypes.byref(oil.c), _d(out), ctypes.byref(l2))
+        if rc:
+            raise RuntimeError("oil project rc=%d" % rc)
         return out, l2.value
 
     def step_adjoint(self, c_n, lam_next, dt, kappa=None, dkappa=None):

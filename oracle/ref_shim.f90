@@ -1,5 +1,5 @@
 ! ref_shim.f90 -- bind(C) entry points onto the reference's own MPI-free
-! leaf modules (knot_vector, gauss, basis, reorderRHS), compiled with flang
+! leaf modules (knot_vector, gauss, basis, reorderRHS, math), compiled with flang
 ! from the sources where they lie.  TEST INFRASTRUCTURE ONLY: the result,
 ! oracle/_ref/libadsref.so, pins the oracle's restatement bitwise; the
 ! product library never links it.
@@ -9,8 +9,27 @@ module ads_ref_shim
    use gauss, only: GaussRule
    use basis, only: BasisData, DersBasisFuns, FindSpan, EvalSpline
    use reorderRHS, only: ReorderRHSForX, ReorderRHSForY, ReorderRHSForZ
+   use math, only: falloff, lerp, bump3d
    implicit none
 contains
+
+   function ref_falloff(r, Rr, t) result(v) bind(C, name="ref_falloff")
+      real(c_double), value :: r, Rr, t
+      real(c_double) :: v
+      v = falloff(r, Rr, t)
+   end function
+
+   function ref_lerp(t, x, y) result(v) bind(C, name="ref_lerp")
+      real(c_double), value :: t, x, y
+      real(c_double) :: v
+      v = lerp(t, x, y)
+   end function
+
+   function ref_bump3d(r, Rr, x, y, z) result(v) bind(C, name="ref_bump3d")
+      real(c_double), value :: r, Rr, x, y, z
+      real(c_double) :: v
+      v = bump3d(r, Rr, x, y, z)
+   end function
 
    subroutine ref_fill_open_knot(n, p, U) bind(C, name="ref_fill_open_knot")
       integer(c_int), value :: n, p

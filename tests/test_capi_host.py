@@ -59,3 +59,11 @@ def test_init_argument_checks_fail_loudly(L):
 def test_null_context_is_rejected(L):
     assert L.ads_step(None, 1e-3, 0, 1, None) == _lib.ADS_ERR_INVALID
     assert L.ads_project(None, 0, None) == _lib.ADS_ERR_INVALID
+
+
+def test_oil_curves_bitwise_equal_oracle(orc, L):
+    for seed, cN, cL in ((1, 30, 16), (7, 3, 4)):
+        a = orc.Oil.random(seed, cN=cN, cL=cL)
+        rhs = adss.OilRHS.with_random_curves(1e-4, seed, cN=cN, cL=cL)
+        assert np.array_equal(rhs.cx, a.cx) and np.array_equal(rhs.cy, a.cy) and np.array_equal(rhs.cz, a.cz)
+    assert L.ads_oil_curves(1, 0, 4, None, None, None) == _lib.ADS_ERR_INVALID
