@@ -33,6 +33,7 @@ ADS_PREC_FP64_KRON = 3
 
 ADS_KERNEL_HEAT_FP64 = 0
 ADS_KERNEL_KRON_RESIDUAL = 1
+ADS_KERNEL_OIL = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -914,8 +914,10 @@ int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double
 int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms_per_launch)
 {
     if (check_ctx(ctx) || !ms_per_launch || launches < 1) return ADS_ERR_INVALID;
-    if (which != ADS_KERNEL_HEAT_FP64 && which != ADS_KERNEL_KRON_RESIDUAL)
+    if (which != ADS_KERNEL_HEAT_FP64 && which != ADS_KERNEL_KRON_RESIDUAL && which != ADS_KERNEL_OIL)
         return fail(ctx, ADS_ERR_INVALID, "ads_time_kernel: unknown kernel");
+    if (which == ADS_KERNEL_OIL && !ctx->oil.ready)
+        return fail(ctx, ADS_ERR_INVALID, "ads_time_kernel: call ads_oil_setup first");
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
     if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
@@ -934,6 +936,9 @@ int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms
         if (which == ADS_KERNEL_HEAT_FP64)
             e = ads::launch_assemble<double>(ctx->p, ads::kFormHeat, 0, ctx->c, ctx->mu, ctx->tb64, dt, ctx->kappa,
                                              ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream);
+        else if (which == ADS_KERNEL_OIL)
+            e = ads::launch_oil(ctx->p, false, ctx->c, ctx->mu, ctx->tb64, ctx->oil.dev, ctx->oil.kq, dt,
+                                ctx->l2_part, ctx->oil.dr_part, ctx->l2_cap, &nparts, ctx->stream);
         else
             e = ads::launch_kron_residual<float, float>(ctx->p, ctx->c, nullptr, nullptr, ctx->f32b, bs, dt,
                                                         ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream);

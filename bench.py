@@ -23,6 +23,10 @@ runs on, bracketed by device synchronisation).  Alongside it:
     alone with HIP events on the library stream (ads_time_kernel), its
     algorithmic FP64 flops against MI355X's FP64 peak; ``kron_residual`` the
     same for the fused residual kernel of the fast precision form, by bytes;
+  * ``oil``: the oil problem (problems/oil: kq(X) exp(mi u) permeability,
+    pumps, drains) at 128^3 elements, p = 2: step time, its point-quadrature
+    kernel against the FP64 peak, and the oracle's faithful step on a 48^3
+    sample on the host cores;
   * ``cpu_baseline``: the oracle's reference-structured CPU path (per-point
     integrand loops, rebuilt 1-D factorisations, per-line solves) on a
     bounded sample of the same step, extrapolated to DOF/s.
@@ -59,6 +63,39 @@ def residual_bytes(D):
     """Algorithmic HBM bytes of the first residual pass: read c (FP64) and
     write r (FP32) once: 12 B/DOF."""
     return 12 * D
+
+
+OIL_PUMPS = [(0.3, 0.3, 0.5), (0.7, 0.7, 0.5)]
+OIL_DRAINS = [(0.5, 0.5, 0.2), (0.5, 0.5, 0.8)]
+
+
+def oil_kernel_flops(E, P):
+    """Algorithmic flops of one oil right-hand-side launch per element: the
+    sum-factorised FormUn (u and grad u at Q^3 = P^3 points: x 2P^4, y 3P^4,
+    z 4P^4 FMA) and the transposed contraction of the four point weights
+    (z 4P^4, y 3P^4, x 2P^4): 18 P^4 FMA = 36 P^4 flops (the per-point
+    weights, exp and source sums not counted)."""
+    return E * 36 * P ** 4
+
+
+def oil_cpu_baseline(rhs, dt):
+    """The oracle's faithful oil step (per-point integrand loops, rebuilt 1-D
+    factorisations) at 48^3 elements, p = 2, same curves, pumps and drains."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    n, p = 49, 2
+    S = O.Setup(n, p)
+    mi, kmin, kmax, rad, ps, ds = rhs.params
+    oil = O.Oil(rhs.cx, rhs.cy, rhs.cz, rhs.cN, rhs.cL, pumps=rhs.pumps, drains=rhs.drains, mi=mi, kq_min=kmin,
+                kq_max=kmax, radius=rad, pump_strength=ps, drain_strength=ds)
+    kq = S.oil_kq_cache(oil)
+    c, _ = S.oil_project(oil)
+    t0 = time.perf_counter()
+    S.oil_step(oil, kq, c, dt)
+    t = time.perf_counter() - t0
+    return {"value": S.D / t, "unit": "DOF/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "port", "sample": "oracle faithful oil step at 48^3 elements, p = 2 (%.2f s)" % t}
 
 
 def cpu_baseline(n, p, dt, budget_s=8.0):
@@ -100,6 +137,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-oil", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: no CPU baseline, no gradient, plus one cold 1 GiB device "
                          "copy (known bytes) to calibrate FETCH_SIZE/WRITE_SIZE")
@@ -241,6 +279,48 @@ def main():
                 "forward_ms": 1e3 * t_fwd, "gradient_ms": 1e3 * t_bwd, "ratio": t_bwd / t_fwd}
         solver.close()
 
+    # ---- oil problem at 128^3, p = 2 ---------------------------------------
+    oil = None
+    if not args.no_oil:
+        # explicit-step stability: dt Kqmax exp(mi u) lambda_max(M^-1 K) < 2 at h = 1/128
+        n5, p5, dt5 = 129, 2, 1e-9
+        ads5, data5 = adss.initialize(n5, p5)
+        h5 = data5.handle
+        L.ads_set_stream(h5, ctypes.c_void_p(stream.cuda_stream))
+        rhs5 = adss.OilRHS.with_random_curves(dt5, 1, pumps=OIL_PUMPS, drains=OIL_DRAINS)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rhs5.setup(data5)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        adss.Step(0, rhs5, ads5, data5)
+        for _ in range(args.warmup):
+            _lib.check(L.ads_oil_step(h5, dt5, None, None), h5)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            _lib.check(L.ads_oil_step(h5, dt5, None, None), h5)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_oil = e0.elapsed_time(e1) / args.steps
+        l2o = ctypes.c_double()
+        _lib.check(L.ads_last_l2norm(h5, ctypes.byref(l2o)), h5)
+        _lib.check(L.ads_time_kernel(h5, _lib.ADS_KERNEL_OIL, dt5, 2, ctypes.byref(ms_k)), h5)
+        _lib.check(L.ads_time_kernel(h5, _lib.ADS_KERNEL_OIL, dt5, args.steps, ctypes.byref(ms_k)), h5)
+        E5 = 128 ** 3
+        D5 = data5.D
+        fl5 = oil_kernel_flops(E5, p5 + 1)
+        ach5 = fl5 / (ms_k.value * 1e-3) / 1e12
+        oil = {"workload": "oil problem, 128x128x128 elements, p=2, 130^3 DOFs, dt=1e-9, 30 curves x 16 points "
+                           "(seed 1), 2 pumps, 2 drains, mi=10, Kq 1..1000, radius 0.15",
+               "value": D5 / (ms_oil * 1e-3), "unit": "DOF/s", "ms_per_step": ms_oil, "l2norm": l2o.value,
+               "kq_setup_ms": 1e3 * t_setup,
+               "kernel": {"name": "k_oil<P=3> (FP64)", "bound": "mfma", "ms_per_launch": ms_k.value,
+                          "flops_per_launch": fl5, "achieved": ach5, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                          "frac": ach5 / PEAK_FP64_TFLOPS},
+               "cpu_baseline": None if args.no_cpu_baseline else oil_cpu_baseline(rhs5, dt5)}
+        adss.Cleanup(ads5, data5)
+
     line = {
         "metric": "ADS step DOF/s, 256^3 p=3, FP64 vs FP32+IR at <=1e-10 rel; grad/fwd time ratio",
         "value": D / (ms64 * 1e-3),
@@ -263,6 +343,7 @@ def main():
         "fp32_ir_kron": forms["fp32_ir_kron"],
         "fp64_kron": forms["fp64_kron"],
         "grad_fwd": grad,
+        "oil": oil,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "k_heat<double,P=4> (FP64 VALU; MI355X's FP64 vector and matrix peaks are equal)",

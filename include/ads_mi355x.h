@@ -46,6 +46,7 @@ extern "C" {
 /* kernels for ads_time_kernel */
 #define ADS_KERNEL_HEAT_FP64 0      /* heat RHS assembly, FP64 */
 #define ADS_KERNEL_KRON_RESIDUAL 1  /* fused FP64 Kronecker residual, first pass (x = 0, FP32 out, l2) */
+#define ADS_KERNEL_OIL 2            /* oil right-hand side (after ads_oil_setup) */
 
 typedef struct ads_ctx ads_ctx;
 
