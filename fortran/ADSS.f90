@@ -6,6 +6,8 @@
 ! its callback to a device form once, after initialize:
 !     call RegisterHeatForm(ComputePointForRHS, Dt, ADS_F0_BUMP3D, mierr)
 !     call RegisterProjectionForm(ComputePointForRHS, ADS_F0_CONST, mierr)
+!     call RegisterOilForm(ComputePointForRHS, Dt, ads, cx, cy, cz, cN, cL, &
+!                          pumps, drains, mi, Kqmin, Kqmax, radius, ps, ds, mierr)
 ! and Step looks the procedure up by address (c_funloc).  An unregistered
 ! procedure makes Step return mierr = ADS_ERR_UNSUPPORTED.
 module ADSS
@@ -17,12 +19,12 @@ module ADSS
    private
 
    public :: initialize, Step, MultiStep, Cleanup, PrintSolution
-   public :: RegisterHeatForm, RegisterProjectionForm, SyncSolution, SetDiffusivity
-   public :: ads_sync_every_step
+   public :: RegisterHeatForm, RegisterProjectionForm, RegisterOilForm, SyncSolution, SetDiffusivity
+   public :: ads_sync_every_step, oil_drained
    public :: ADS_F0_CONST, ADS_F0_SINPROD, ADS_F0_BUMP3D, ADS_PREC_FP64, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_KRON, &
              ADS_PREC_FP64_KRON
 
-   integer, parameter :: FORM_HEAT = 1, FORM_PROJECTION = 2, MAXFORMS = 16
+   integer, parameter :: FORM_HEAT = 1, FORM_PROJECTION = 2, FORM_OIL = 3, MAXFORMS = 16
 
    type form_entry
       type(c_funptr) :: fptr = c_null_funptr
@@ -40,6 +42,10 @@ module ADSS
    ! reference's data contract); set .false. and call SyncSolution to read
    ! them only when needed
    logical, save :: ads_sync_every_step = .true.
+
+   ! the oil form's running total of J W v Dt vdrain over its steps (the
+   ! problem's module variable `drained`, problems/oil/RHS_eq.F90:86)
+   real(kind=8), save :: oil_drained = 0.d0
 
 contains
 
@@ -160,6 +166,50 @@ contains
       call add_form(RHS_fun, FORM_PROJECTION, f0, 0.d0, ADS_PREC_FP64, 1_c_int, mierr)
    end subroutine
 
+   ! Binds the oil integrand (problems/oil/RHS_eq.F90:35-94) to the device
+   ! form and does InitInputData's and PrecomputeKq's work on the device:
+   ! curves cx, cy, cz (cN curves of cL points, curve i at (i-1)*cL+1..i*cL),
+   ! pumps(3, np) and drains(3, nd) as InitPumps reads them, and the
+   ! input_data constants.  Step(0, ...) projects initial_state, Step(iter >
+   ! 0, ...) takes one explicit step of length Dt and adds its drained to
+   ! oil_drained.
+   subroutine RegisterOilForm(RHS_fun, Dt, ads, cx, cy, cz, cN, cL, pumps, drains, mi, Kqmin, Kqmax, radius, &
+                              pumping_strength, draining_strength, mierr)
+      procedure(RHS_fun_int) :: RHS_fun
+      real(kind=8), intent(in) :: Dt
+      type(ADS_setup), intent(in) :: ads
+      real(kind=8), intent(in) :: cx(:), cy(:), cz(:), pumps(:, :), drains(:, :)
+      integer(kind=4), intent(in) :: cN, cL
+      real(kind=8), intent(in) :: mi, Kqmin, Kqmax, radius, pumping_strength, draining_strength
+      integer(kind=4), intent(out) :: mierr
+      real(c_double) :: prm(6)
+      real(c_double) :: none(3)
+      prm = (/ mi, Kqmin, Kqmax, radius, pumping_strength, draining_strength /)
+      none = 0.d0
+      if (size(pumps, 1) /= 3 .or. size(drains, 1) /= 3 .or. size(cx) < cN * cL .or. size(cy) < cN * cL &
+          .or. size(cz) < cN * cL) then
+         mierr = ADS_ERR_INVALID
+         write(*, '(a)') 'RegisterOilForm: curves need cN*cL points, pumps and drains 3 rows'
+         return
+      end if
+      if (size(pumps, 2) > 0 .and. size(drains, 2) > 0) then
+         mierr = ads_oil_setup(ads%handle, cx, cy, cz, int(cN, c_int), int(cL, c_int), pumps, &
+                               int(size(pumps, 2), c_int), drains, int(size(drains, 2), c_int), prm)
+      else if (size(pumps, 2) > 0) then
+         mierr = ads_oil_setup(ads%handle, cx, cy, cz, int(cN, c_int), int(cL, c_int), pumps, &
+                               int(size(pumps, 2), c_int), none, 0_c_int, prm)
+      else if (size(drains, 2) > 0) then
+         mierr = ads_oil_setup(ads%handle, cx, cy, cz, int(cN, c_int), int(cL, c_int), none, 0_c_int, &
+                               drains, int(size(drains, 2), c_int), prm)
+      else
+         mierr = ads_oil_setup(ads%handle, cx, cy, cz, int(cN, c_int), int(cL, c_int), none, 0_c_int, &
+                               none, 0_c_int, prm)
+      end if
+      call report(ads%handle, mierr, 'RegisterOilForm')
+      if (mierr /= ADS_OK) return
+      call add_form(RHS_fun, FORM_OIL, ADS_F0_CONST, real(Dt, c_double), ADS_PREC_FP64, 1_c_int, mierr)
+   end subroutine
+
    ! per-element diffusivity (nelem(1)*nelem(2)*nelem(3), ex fastest)
    subroutine SetDiffusivity(ads, kappa, mierr)
       type(ADS_setup), intent(in) :: ads
@@ -187,7 +237,7 @@ contains
       integer(kind=4), intent(out) :: mierr
       type(c_funptr) :: fp
       integer :: i
-      real(c_double) :: l2
+      real(c_double) :: l2, dr
       integer(c_int) :: rc
       fp = c_funloc(RHS_fun)
       l2norm = 0.d0
@@ -195,11 +245,20 @@ contains
          if (c_associated(forms(i)%fptr, fp)) exit
       end do
       if (i > nforms) then
-         write(*, '(a)') 'Step: RHS_fun is not bound to a device form (RegisterHeatForm/RegisterProjectionForm)'
+         write(*, '(a)') 'Step: RHS_fun is not bound to a device form (RegisterHeatForm/RegisterOilForm/...)'
          mierr = ADS_ERR_UNSUPPORTED
          return
       end if
-      if (forms(i)%kind == FORM_PROJECTION .or. iter == 0) then
+      if (forms(i)%kind == FORM_OIL) then
+         if (iter == 0) then
+            rc = ads_oil_project(ads%handle, l2)
+            ads_data%t = 0.d0
+         else
+            rc = ads_oil_step(ads%handle, forms(i)%dt, l2, dr)
+            if (rc == ADS_OK) oil_drained = oil_drained + dr
+            ads_data%t = ads_data%t + forms(i)%dt
+         end if
+      else if (forms(i)%kind == FORM_PROJECTION .or. iter == 0) then
          rc = ads_project(ads%handle, forms(i)%f0, l2)
          ads_data%t = 0.d0
       else

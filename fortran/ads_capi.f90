@@ -90,6 +90,24 @@ module ads_capi
          real(c_double), value :: dt
          real(c_double), intent(out) :: l2norm
       end function
+      integer(c_int) function ads_oil_setup(ctx, cx, cy, cz, cN, cL, pumps, npumps, drains, ndrains, params) &
+            bind(C, name="ads_oil_setup")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: cx(*), cy(*), cz(*), pumps(*), drains(*), params(6)
+         integer(c_int), value :: cN, cL, npumps, ndrains
+      end function
+      integer(c_int) function ads_oil_project(ctx, l2norm) bind(C, name="ads_oil_project")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: l2norm
+      end function
+      integer(c_int) function ads_oil_step(ctx, dt, l2norm, drained) bind(C, name="ads_oil_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: l2norm, drained
+      end function
    end interface
 
 contains
