@@ -97,6 +97,12 @@ module ads_capi
          real(c_double), intent(in) :: cx(*), cy(*), cz(*), pumps(*), drains(*), params(6)
          integer(c_int), value :: cN, cL, npumps, ndrains
       end function
+      integer(c_int) function ads_oil_curves(seed, cN, cL, cx, cy, cz) bind(C, name="ads_oil_curves")
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: seed
+         integer(c_int), value :: cN, cL
+         real(c_double), intent(out) :: cx(*), cy(*), cz(*)
+      end function
       integer(c_int) function ads_oil_project(ctx, l2norm) bind(C, name="ads_oil_project")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx

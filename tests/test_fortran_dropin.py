@@ -80,3 +80,36 @@ def test_fortran_heat_run_matches_oracle(driver, orc, tmp_path):
                                                   orc._d(cf), x, y, z)
                         for z in t for y in t for x in t])
         assert np.max(np.abs(vals - exp)) <= 6e-11
+
+
+@pytest.mark.gpu
+def test_fortran_oil_run_matches_oracle(orc, tmp_path):
+    """oil_driver 9 2 3 1e-7 seed 1, one pump, one drain (8^3 elements,
+    p = 2; the oil problem's constants): per-step l2norm, final coefficients
+    and the drained total vs the oracle with the same (seeded) curves."""
+    drv = os.path.join(ROOT, "fortran", "_build", "oil_driver")
+    if not os.path.exists(drv):
+        pytest.skip("fortran/_build/oil_driver not built (needs flang)")
+    out = tmp_path / "c.bin"
+    pump, drain = (0.5, 0.45, 0.55), (0.55, 0.5, 0.5)
+    args = [drv, "9", "2", "3", "1e-7", "1", "1", *map(str, pump), "1", *map(str, drain), str(out)]
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(res.stdout, res.stderr, flush=True)
+    assert res.returncode == 0
+    l2 = [float(line.split()[-1]) for line in res.stdout.splitlines() if "L2 norm" in line]
+    drained = [float(line.split()[-1]) for line in res.stdout.splitlines() if line.startswith("drained:")][0]
+    S = orc.Setup(9, 2)
+    oil = orc.Oil.random(1, pumps=[pump], drains=[drain])
+    kq = S.oil_kq_cache(oil)
+    c, l2o = S.oil_project(oil)
+    ref, dr_tot = [l2o], 0.0
+    for _ in range(3):
+        c, l2s, dr = S.oil_step(oil, kq, c, 1e-7)
+        ref.append(l2s)
+        dr_tot += dr
+    assert len(l2) == 4 and ref[0] > 0
+    for a, b in zip(l2, ref):
+        assert abs(a - b) <= 1e-12 * abs(b)
+    got = np.fromfile(out, dtype=np.float64)
+    assert np.linalg.norm(got - c) <= 1e-12 * np.linalg.norm(c)
+    assert abs(drained - dr_tot) <= 1e-12 * max(abs(dr_tot), 1e-300)
