@@ -499,15 +499,21 @@ __device__ double oil_dist_curves(const OilDev& o, double px, double py, double 
     return fval;
 }
 
-// sum over the sources at (x, y, z) of strength * falloff(0, radius, |q - X|)
+// sum over the sources at (x, y, z) of strength * falloff(0, radius, |q - X|).
+// A source with |q - X|^2 > radius^2 (1 + 1e-12) has sqrt > radius after
+// rounding, so falloff is exactly 0 and adding its term leaves the sum as
+// it is: such sources are skipped before the square root and the division.
 __device__ __forceinline__ double oil_sources(const double* __restrict__ q, int nq, double strength, double radius,
                                               double x, double y, double z)
 {
 #pragma clang fp contract(off)
+    const double reject = radius * radius * (1.0 + 1e-12);
     double f = 0.0;
     for (int i = 0; i < nq; ++i) {
         const double dx = q[3 * i] - x, dy = q[3 * i + 1] - y, dz = q[3 * i + 2] - z;
-        f = f + strength * oil_falloff(0.0, radius, sqrt(dx * dx + dy * dy + dz * dz));
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > reject) continue;
+        f = f + strength * oil_falloff(0.0, radius, sqrt(d2));
     }
     return f;
 }
