@@ -4,7 +4,7 @@
   * against the oracle's FP64 adjoint (reference gradient) at a size the CPU
     finishes in seconds, <= 1e-11 relative;
   * against central finite differences (eps = 1e-6) on 32 random directions
-    at the config 4 size (96^3 elements, p = 2, 50 steps, checkpoint 10,
+    at 96^3 elements, p = 2 (50 steps, checkpoint 10,
     lognormal(0, 0.5) diffusivity from seed 3, loss 0.5 ||c_T - c_obs||^2 with
     c_obs a forward run under a seed-4 diffusivity).  The directions are
     standard normal in every component (not normalised): a unit vector over
@@ -66,9 +66,9 @@ def test_gradient_vs_oracle_adjoint(orc):
         solver.close()
 
 
-def test_gradient_vs_finite_differences_config4():
+def test_gradient_vs_finite_differences_96cubed_p2():
     n, p, dt, steps, ck = 97, 2, 1e-7, 50, 10
-    say("case FD config4")
+    say("case FD 96^3 p=2")
     solver = HeatSolver(n, p)
     try:
         D, E = solver.D, solver.E
@@ -113,13 +113,13 @@ def test_gradient_vs_finite_differences_config4():
         solver.close()
 
 
-def test_config5_fp32_forms_gradient():
+def test_fp32_forms_gradient_384x384x256_p3():
     """384x384x256 elements, p = 3, lognormal(0, 0.5) diffusivity from seed 5,
     40 steps, checkpoint 8, dt = 5e-9 (stable for the diffusivity's range):
     the forward in FP32 + refinement and the adjoint in FP32 with FP64
     mass-residual corrections, against the all-FP64 rollout and gradient."""
     n, p, dt, steps, ck = (386, 386, 258), 3, 5e-9, 40, 8
-    say("case config5")
+    say("case fp32 forms gradient 384x384x256 p=3")
     solver = HeatSolver(n, p)
     try:
         D, E = solver.D, solver.E
@@ -140,7 +140,7 @@ def test_config5_fp32_forms_gradient():
         e_state = float(torch.linalg.norm(res["fp32"][0] - res["fp64"][0]) / torch.linalg.norm(res["fp64"][0]))
         e_gc = float(torch.linalg.norm(res["fp32"][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))
         e_gk = float(torch.linalg.norm(res["fp32"][2] - res["fp64"][2]) / torch.linalg.norm(res["fp64"][2]))
-        say("config5 rel err: state", e_state, "grad c0", e_gc, "grad kappa", e_gk)
+        say("rel err: state", e_state, "grad c0", e_gc, "grad kappa", e_gk)
         # each refined step is within 1e-10 of the FP64 step and the step is a
         # contraction, so 40 steps stay within 40 x 1e-10
         assert e_state <= 4e-9

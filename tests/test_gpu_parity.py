@@ -130,8 +130,8 @@ def test_solve_vs_oracle(orc, n, p):
         C.close()
 
 
-def test_config1_heat_run():
-    """Config 1: 16^3 elements, p = 2 (SIZE = 17), dt = 1e-5, 10 steps after
+def test_heat_run_16cubed_p2():
+    """16^3 elements, p = 2 (SIZE = 17), dt = 1e-5, 10 steps after
     the t = 0 projection of prod sin(pi x_i) plus U[-0.01, 0.01) noise (seed 0).
     Every step is compared with the oracle's faithful path (the reference's
     per-point loops, rebuilt 1-D factorisations)."""
@@ -141,7 +141,7 @@ def test_config1_heat_run():
     S = O.Setup(n, p)
     ads, data = adss.initialize(n, p)
     try:
-        say("case config1 projection")
+        say("case heat 16^3 p=2 projection")
         rhs = adss.HeatRHS(Dt=dt, f0=adss.ADS_F0_SINPROD)
         l2 = adss.Step(0, rhs, ads, data)
         c0, l2o = S.project(O.F0_SINPROD, O.MODE_FAITHFUL)
@@ -152,7 +152,7 @@ def test_config1_heat_run():
         data.F = c
         rhs.t += dt
         for it in range(1, steps + 1):
-            say("case config1 step", it)
+            say("case heat 16^3 p=2 step", it)
             l2 = adss.Step(it, rhs, ads, data)
             c, l2o = S.step(c, dt, mode=O.MODE_FAITHFUL)
             assert rel(data.F, c) <= TOL, it
@@ -162,8 +162,8 @@ def test_config1_heat_run():
         adss.Cleanup(ads, data)
 
 
-def test_config2_steps():
-    """Config 2 size (128^3 elements, p = 2, 130^3 DOFs, dt = 1e-6, seed 1):
+def test_heat_steps_128cubed_p2():
+    """128^3 elements, p = 2, 130^3 DOFs, dt = 1e-6, seed 1:
     projection + noise + 2 steps against the oracle (sum-factorised)."""
     import oracle as O
 
@@ -171,7 +171,7 @@ def test_config2_steps():
     S = O.Setup(n, p)
     ads, data = adss.initialize(n, p)
     try:
-        say("case config2 projection")
+        say("case heat 128^3 p=2 projection")
         rhs = adss.HeatRHS(Dt=dt, f0=adss.ADS_F0_SINPROD)
         adss.Step(0, rhs, ads, data)
         c0, _ = S.project(O.F0_SINPROD)
@@ -180,7 +180,7 @@ def test_config2_steps():
         data.F = c
         rhs.t = dt
         for it in (1, 2):
-            say("case config2 step", it)
+            say("case heat 128^3 p=2 step", it)
             l2 = adss.Step(it, rhs, ads, data)
             c, l2o = S.step(c, dt)
             assert rel(data.F, c) <= TOL
