@@ -30,6 +30,7 @@ ADS_PREC_FP64 = 0
 ADS_PREC_FP32_IR = 1
 ADS_PREC_FP32_IR_KRON = 2
 ADS_PREC_FP64_KRON = 3
+ADS_PREC_FP32_IR_SEG = 4
 
 ADS_KERNEL_HEAT_FP64 = 0
 ADS_KERNEL_KRON_RESIDUAL = 1
@@ -101,6 +102,7 @@ def load(path: str | None = None):
     L.ads_oil_project.argtypes = [_vp, _dp]
     L.ads_oil_step.argtypes = [_vp, ctypes.c_double, _dp, _dp]
     L.ads_oil_get_kq.argtypes = [_vp, _dp]
+    L.ads_host_seg_tables.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_float)]
     L.ads_oil_curves.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
     L.ads_fill_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_fill_lognormal.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]

@@ -32,13 +32,13 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (ADS_F0_BUMP3D, ADS_F0_CONST, ADS_F0_SINPROD, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_KRON,
-                   ADS_PREC_FP64, ADS_PREC_FP64_KRON, AdsError, check)
+from ._lib import (ADS_F0_BUMP3D, ADS_F0_CONST, ADS_F0_SINPROD, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_SEG,
+                   ADS_PREC_FP32_IR_KRON, ADS_PREC_FP64, ADS_PREC_FP64_KRON, AdsError, check)
 
 __all__ = [
     "ADS_setup", "ADS_compute_data", "initialize", "Step", "MultiStep", "Cleanup", "Form3DRHS", "FormUn",
     "ComputeMatrix", "HeatRHS", "OilRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
-    "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR", "ADS_PREC_FP32_IR_KRON", "ADS_PREC_FP64_KRON", "PlotParams", "EvalGrid", "VtkOutput",
+    "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR", "ADS_PREC_FP32_IR_KRON", "ADS_PREC_FP64_KRON", "ADS_PREC_FP32_IR_SEG", "PlotParams", "EvalGrid", "VtkOutput",
     "PrintSolution",
 ]
 

@@ -77,6 +77,11 @@ struct ads_ctx {
 
     std::vector<MixSet> mixsets;  // MultiStep operators factored so far
 
+    // band-inverse form: FP32 band tables of the mass-matrix inverses (built on first use)
+    ads::SegLU seg[3]{};
+    float* f32c = nullptr;
+    float* f32d = nullptr;
+
     // the oil problem (ads_oil_setup)
     struct Oil {
         double* curves = nullptr;
@@ -185,6 +190,41 @@ int ensure_ir_scratch(ads_ctx* ctx)
     return ADS_OK;
 }
 
+// the segmented form's factor tables, built on first use: rows matching the
+// middle one to 1e-7 use it, and the warm-up w covers a decay to 1e-8
+constexpr double kSegTol = 1e-7, kSegTolW = 1e-8;
+
+int ensure_seg(ads_ctx* ctx)
+{
+    int rc;
+    if (!ctx->f32c && (rc = dev_alloc(ctx, &ctx->f32c, ctx->D))) return rc;
+    if (!ctx->f32d && (rc = dev_alloc(ctx, &ctx->f32d, ctx->D))) return rc;
+    for (int d = 0; d < 3; ++d) {
+        if (ctx->seg[d].mid) continue;
+        std::vector<float> tab;
+        int w = 0, lo = 0, hi = 0;
+        const std::string e = ads::seg_lu_tables(ctx->dir[d], kSegTol, kSegTolW, ads::kSegPadLo, tab, w, lo, hi);
+        if (!e.empty()) return fail(ctx, ADS_ERR_UNSUPPORTED, "segmented form, direction " + std::to_string(d) + ": " + e);
+        // device layout: the middle row, kSegPadLo zero rows, rows 0..n-1, kSegPadHi zero rows
+        const int n = ctx->dir[d].dofs, nt = 2 * ctx->p + 1;
+        std::vector<float> dev_tab(static_cast<size_t>(1 + ads::kSegPadLo + n + ads::kSegPadHi) * nt, 0.0f);
+        std::copy(tab.begin() + static_cast<long>(n) * nt, tab.end(), dev_tab.begin());
+        std::copy(tab.begin(), tab.begin() + static_cast<long>(n) * nt,
+                  dev_tab.begin() + static_cast<long>(1 + ads::kSegPadLo) * nt);
+        float* dev = nullptr;
+        if ((rc = dev_alloc(ctx, &dev, dev_tab.size()))) return rc;
+        ADS_HIP(ctx, hipMemcpy(dev, dev_tab.data(), sizeof(float) * dev_tab.size(), hipMemcpyHostToDevice));
+        ctx->seg[d] = ads::SegLU{dev, dev + static_cast<long>(1 + ads::kSegPadLo) * nt, n, lo, hi, w, 0};
+    }
+    return ADS_OK;
+}
+
+int seg_solve(ads_ctx* ctx, const float* in, float* tmp, float* out, const ads::SegOut& fo = ads::SegOut{})
+{
+    ADS_HIP(ctx, ads::launch_seg_solve(ctx->p, in, tmp, out, ctx->seg, fo, ctx->stream));
+    return ADS_OK;
+}
+
 __global__ void k_fill_zero_d(double* p, long n)
 {
     const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -248,14 +288,15 @@ ads::BandSet band_set(const ads_ctx* ctx)
 }
 
 // r = b + A c - M x in FP64 (nullptr operands are zero), rounded to TR;
-// with l2_d also l2 = c^T M c -> l2_d.
+// with l2_d also l2 = c^T M c -> l2_d; with bo also r in FP64 -> bo.
 template <typename TX, typename TR = float>
-int residual(ads_ctx* ctx, const double* c, const TX* x, const double* b, TR* r, double dtk, double* l2_d)
+int residual(ads_ctx* ctx, const double* c, const TX* x, const double* b, TR* r, double dtk, double* l2_d,
+             double* bo = nullptr)
 {
     const ads::BandSet bs = band_set(ctx);
     int nres = 0;
     ADS_HIP(ctx, (ads::launch_kron_residual<TX, TR>(ctx->p, c, x, b, r, bs, dtk, l2_d ? ctx->l2_part : nullptr,
-                                                    ctx->l2_cap, &nres, ctx->stream)));
+                                                    ctx->l2_cap, &nres, ctx->stream, bo)));
     if (l2_d) ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nres, l2_d, ctx->stream));
     return ADS_OK;
 }
@@ -322,41 +363,74 @@ int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa
 }
 
 // c_out = M^-1 A(kappa) c_in from FP64 residuals and FP32 sweeps, `refine`
-// sweeps in all.  Constant kappa: each pass computes the Kronecker residual
-// r = A c - M x (the first, x = 0, also gives l2 = c^T M c); the first
+// sweeps in all.  Constant kappa: the first pass computes the Kronecker
+// right-hand side b = A c (the residual at x = 0; also l2 = c^T M c), kept
+// in FP64, and the later ones the mass residual b - M x; the first
 // correction d1 is FP32, so the second residual reads it as FP32 and
 // x = d1 + d2 is formed in FP64 once.  Per-element kappa: b = A(kappa) c
-// from the FP64 element assembly, then x = M^-1 b by mass_solve_ir.
+// from the FP64 element assembly, then x = M^-1 b by mass_solve_ir.  c_in is
+// not read once b is formed, so x accumulates in c_out even in place.
 int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa_d, double dt, int refine,
                  double* l2_d)
 {
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
-    double* x = c_out != c_in ? c_out : ctx->x64;
+    double* x = c_out;
     if (kappa_d) {
         if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in, ctx->b64, dt, kappa_d, l2_d))) return rc;
         if ((rc = mass_solve_ir(ctx, ctx->b64, x, nullptr, refine - 1, false))) return rc;
     } else {
         float* d1 = ctx->f32a;
         float* d2 = ctx->f32b;
-        if ((rc = residual<float>(ctx, c_in, nullptr, nullptr, d1, dt, l2_d))) return rc;
+        // the first residual (x = 0) is b = A c: kept in FP64 for the mass
+        // residuals b - M x of the later passes
+        double* b = refine > 1 ? ctx->b64 : nullptr;
+        if ((rc = residual<float>(ctx, c_in, nullptr, nullptr, d1, dt, l2_d, b))) return rc;
         ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, d1, ctx->sw32, false, ctx->stream));
         if (refine == 1) {
             ADS_HIP(ctx, ads::launch_add_f2d(d1, nullptr, x, ctx->D, ctx->stream));
         } else {
-            if ((rc = residual<float>(ctx, c_in, d1, nullptr, d2, dt, nullptr))) return rc;
+            if ((rc = residual<float>(ctx, nullptr, d1, b, d2, 0.0, nullptr))) return rc;
             ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, d2, ctx->sw32, false, ctx->stream));
             ADS_HIP(ctx, ads::launch_add_f2d(d1, d2, x, ctx->D, ctx->stream));
             for (int k = 2; k < refine; ++k) {
-                if ((rc = residual<double>(ctx, c_in, x, nullptr, d2, dt, nullptr))) return rc;
+                if ((rc = residual<double>(ctx, nullptr, x, b, d2, 0.0, nullptr))) return rc;
                 ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, d2, ctx->sw32, false, ctx->stream));
                 ADS_HIP(ctx, ads::launch_add_f2d(nullptr, d2, x, ctx->D, ctx->stream));
             }
         }
     }
-    if (x != c_out)
-        ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
     return ADS_OK;
+}
+
+// The segmented form: as step_ir_kron with every FP32 solve replaced by the
+// segmented solve, whose last direction also forms x in FP64 (a per-element
+// kappa takes step_ir_kron's path).
+int step_ir_seg(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa_d, double dt, int refine,
+                double* l2_d)
+{
+    if (kappa_d) return step_ir_kron(ctx, c_in, c_out, kappa_d, dt, refine, l2_d);
+    int rc = ensure_ir_scratch(ctx);
+    if (rc || (rc = ensure_seg(ctx))) return rc;
+    double* x = c_out;  // c_in is not read once b is formed
+    float* r = ctx->f32a;
+    float* tmp = ctx->f32b;
+    float* d1 = ctx->f32c;
+    float* mid = ctx->f32d;  // the x-direction intermediate of the later solves
+    double* b = refine > 1 ? ctx->b64 : nullptr;
+    if ((rc = residual<float>(ctx, c_in, nullptr, nullptr, r, dt, l2_d, b))) return rc;
+    if (refine == 1) {
+        rc = seg_solve(ctx, r, tmp, d1, ads::SegOut{x, nullptr, ads::kSegSet});
+    } else {
+        if ((rc = seg_solve(ctx, r, tmp, d1))) return rc;
+        if ((rc = residual<float>(ctx, nullptr, d1, b, r, 0.0, nullptr))) return rc;
+        if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, d1, ads::kSegAdd}))) return rc;
+        for (int k = 2; k < refine && !rc; ++k) {
+            if ((rc = residual<double>(ctx, nullptr, x, b, r, 0.0, nullptr))) return rc;
+            rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, nullptr, ads::kSegAccumulate});
+        }
+    }
+    return rc;
 }
 
 // device copy of 3 n values (three arrays of n, or n triples when b and c
@@ -628,12 +702,13 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
             ADS_HIP(ctx, hipMemcpyAsync(c_out_d, target, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
         return ADS_OK;
     }
-    if (prec == ADS_PREC_FP32_IR || prec == ADS_PREC_FP32_IR_KRON) {
+    if (prec == ADS_PREC_FP32_IR || prec == ADS_PREC_FP32_IR_KRON || prec == ADS_PREC_FP32_IR_SEG) {
         if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
         if (c_in_d == c_out_d && c_out_d == ctx->x64)
             return fail(ctx, ADS_ERR_INVALID, "internal: buffer aliasing");
-        return prec == ADS_PREC_FP32_IR ? step_ir(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d)
-                                        : step_ir_kron(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d);
+        if (prec == ADS_PREC_FP32_IR) return step_ir(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d);
+        if (prec == ADS_PREC_FP32_IR_KRON) return step_ir_kron(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d);
+        return step_ir_seg(ctx, c_in_d, c_out_d, kappa_d, dt, refine, l2_d);
     }
     return fail(ctx, ADS_ERR_INVALID, "unknown precision form");
 }
@@ -901,6 +976,18 @@ int ads_oil_get_kq(ads_ctx* ctx, double* kq_host)
     ADS_HIP(ctx, hipMemcpyAsync(kq_host, ctx->oil.kq, sizeof(double) * ctx->E * q3, hipMemcpyDeviceToHost,
                                 ctx->stream));
     ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ADS_OK;
+}
+
+int ads_host_seg_tables(int n, int p, int* w, int* lo, int* hi, float* tab)
+{
+    if (!w || !lo || !hi || !tab || n < p || p < 1 || p > 4) return ADS_ERR_INVALID;
+    ads::Direction d;
+    if (!ads::build_direction(n, p, d).empty()) return ADS_ERR_INVALID;
+    std::vector<float> t;
+    if (!ads::seg_lu_tables(d, kSegTol, kSegTolW, ads::kSegPadLo, t, *w, *lo, *hi).empty())
+        return ADS_ERR_UNSUPPORTED;
+    std::copy(t.begin(), t.end(), tab);
     return ADS_OK;
 }
 

@@ -84,7 +84,9 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
 // written in precision TR.  With l2_part != nullptr each workgroup also
 // writes its partial sum of c^T M c; their count goes to *nparts (for
 // launch_reduce_partials), and a launch that would need more than part_cap
-// is refused before anything runs.
+// is refused before anything runs.  With bo != nullptr (x = nullptr only) r
+// is also written to bo in FP64.  Without c (x given) only the M products are formed: the mass
+// residual r = b - M x (no l2 then).
 struct BandSet {
     const double* mass[3];   // row tables: [i][t] = M(i, i-p+t), t = 0..2p, zero outside
     const double* stiff[3];  // [i][t] = K(i, i-p+t)
@@ -92,7 +94,8 @@ struct BandSet {
 };
 template <typename TX, typename TR>
 hipError_t launch_kron_residual(int p, const double* c, const TX* x, const double* b, TR* r, const BandSet& bs,
-                                double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s);
+                                double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s,
+                                double* bo = nullptr);
 // The oil problem's inputs on the device (problems/oil/input_data.F90):
 // curve points (cx, cy, cz of cN curves x cL points, consecutive), pumps and
 // drains (x, y, z each) and the scalar constants.
@@ -112,6 +115,35 @@ hipError_t launch_oil_kq(int p, const Tables<double>& tb, const OilDev& o, doubl
 hipError_t launch_oil(int p, bool tzero, const double* c, double* F, const Tables<double>& tb, const OilDev& o,
                       const double* kq, double dt, double* l2_part, double* dr_part, int part_cap, int* nparts,
                       hipStream_t s);
+// The FP32 solve of the segmented refinement form: out = (Mz (x) My (x)
+// Mx)^-1 in up to the truncation below, direction by direction, each line cut
+// into segments solved independently from w entries on either side (see
+// k_seg_lines).  Per direction, the factors' rows (2p+1 floats each:
+// L(j, j-1..j-p), 1/U(j, j), U(j, j+1..j+p)) at rows[j*(2p+1)], zero padded
+// for j in [-kSegPadLo, 0) and [n, n + kSegPadHi); the middle row at mid,
+// which rows [lo, hi) equal; w a multiple of 16 up to kSegPadLo.  in is left
+// as it is, tmp is scratch of the same size; the factors must have no row
+// interchanges.
+constexpr int kSegMax = 48;      // outputs per lane at most
+constexpr int kSegPadLo = 48;    // largest w
+constexpr int kSegPadHi = kSegMax + kSegPadLo;
+struct SegLU {
+    const float* mid;
+    const float* rows;
+    int n, lo, hi, w;
+    int seg;  // outputs per lane (set at launch)
+};
+// The result goes to out, or, with fo.x set, to fo.x in FP64 (out then
+// holds the x-direction intermediate): kSegSet x = v, kSegAdd x = fo.add + v,
+// kSegAccumulate x += v.
+enum SegMode { kSegSet = 0, kSegAdd = 1, kSegAccumulate = 2 };
+struct SegOut {
+    double* x;
+    const float* add;
+    int mode;
+};
+hipError_t launch_seg_solve(int p, const float* in, float* tmp, float* out, const SegLU sl[3], const SegOut& fo,
+                            hipStream_t s);
 
 // upper bound of the workgroup (partial-sum) count of launch_kron_residual
 long kron_residual_partials_bound(const BandSet& bs);

@@ -13,6 +13,7 @@
 //                 (src/ADS.F90:352-361), including its zero-pivot failure.
 #include "setup.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -307,6 +308,79 @@ void dense_matrix(const Direction& d, const double mix[4], double* out)
                     const int row = d.first_dof[e] + c, col = d.first_dof[e] + dd;
                     out[static_cast<size_t>(row) * N + col] += mix[0] * M + mix[1] * K + mix[2] * B + mix[3] * BT;
                 }
+}
+
+std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, std::vector<float>& tab, int& w,
+                          int& lo, int& hi)
+{
+    const int N = d.dofs, p = d.p, nt = 2 * p + 1;
+    // M = L U without interchanges (M is symmetric positive definite), in
+    // FP64 over M's band: A(i, j) at a[i*nt + p + j - i]
+    std::vector<double> a(static_cast<size_t>(N) * nt, 0.0);
+    auto A = [&](int i, int j) -> double& { return a[static_cast<size_t>(i) * nt + p + j - i]; };
+    for (int j = 0; j < N; ++j)
+        for (int i = std::max(0, j - p); i <= std::min(N - 1, j + p); ++i)
+            A(i, j) = d.mass_band[static_cast<size_t>(j) * nt + p + i - j];
+    for (int k = 0; k < N; ++k) {
+        if (!(A(k, k) > 0.0)) return "the mass matrix is not positive definite";
+        for (int i = k + 1; i <= std::min(N - 1, k + p); ++i) {
+            const double l = A(i, k) / A(k, k);
+            A(i, k) = l;
+            for (int j = k + 1; j <= std::min(N - 1, k + p); ++j) A(i, j) -= l * A(k, j);
+        }
+    }
+    auto at = [&](int i, int j) { return A(i, j); };
+    auto row = [&](int j, std::vector<double>& r) {
+        r.assign(nt, 0.0);
+        for (int s = 1; s <= p; ++s)
+            if (j - s >= 0) r[s - 1] = at(j, j - s);
+        r[p] = 1.0 / at(j, j);
+        for (int s = 1; s <= p; ++s)
+            if (j + s < N) r[p + s] = at(j, j + s);
+    };
+    tab.assign(static_cast<size_t>(N + 1) * nt, 0.0f);
+    std::vector<double> r, m;
+    for (int j = 0; j < N; ++j) {
+        row(j, r);
+        for (int t = 0; t < nt; ++t) tab[static_cast<size_t>(j) * nt + t] = static_cast<float>(r[t]);
+    }
+    const int mid = N / 2;
+    row(mid, m);
+    for (int t = 0; t < nt; ++t) tab[static_cast<size_t>(N) * nt + t] = static_cast<float>(m[t]);
+    double sl = 0.0, su = 0.0;
+    for (int s = 1; s <= p; ++s) {
+        sl = std::max(sl, std::fabs(m[s - 1]));
+        su = std::max(su, std::fabs(m[p + s]));
+    }
+    auto matches = [&](int j) {
+        row(j, r);
+        for (int s = 1; s <= p; ++s)
+            if (std::fabs(r[s - 1] - m[s - 1]) > tol * sl || std::fabs(r[p + s] - m[p + s]) > tol * su) return false;
+        return std::fabs(r[p] - m[p]) <= tol * std::fabs(m[p]);
+    };
+    lo = mid;
+    while (lo > 0 && matches(lo - 1)) --lo;
+    hi = mid + 1;
+    while (hi < N && matches(hi)) ++hi;
+    // decay of the interior recurrences from a unit state
+    int need = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int len = 4 * wmax + 4 * p;
+        std::vector<double> v(len, 0.0);
+        v[0] = 1.0;
+        int last = 0;
+        for (int k = 1; k < len; ++k) {
+            double s = 0.0;
+            for (int t = 1; t <= p; ++t)
+                if (k - t >= 0) s += (pass == 0 ? m[t - 1] : m[p + t] * m[p]) * v[k - t];
+            v[k] = -s;
+            if (std::fabs(v[k]) > tol_w) last = k;
+        }
+        need = std::max(need, last + 1);
+    }
+    w = (need + 15) / 16 * 16;
+    if (w > wmax) return "the factors' recurrences decay too slowly (" + std::to_string(need) + " entries)";
+    return "";
 }
 
 void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>& fwd_piv,

@@ -73,6 +73,21 @@ void sweep_tables(const Direction& d, std::vector<double>& fwd, std::vector<int>
 void sweep_tables(int N, int p, const std::vector<double>& lu, const std::vector<int>& piv, std::vector<double>& fwd,
                   std::vector<int>& fwd_piv, std::vector<double>& bwd);
 
+// Row tables of the mass matrix's LU factors without row interchanges (M is
+// symmetric positive definite; factored here in FP64 from mass_band) for the
+// segmented FP32 solve: tab[j*(2p+1) + t] = L(j, j-1-t) for t < p (L unit
+// lower), 1/U(j, j) at t = p, U(j, j+t-p) for t > p; zero outside the
+// matrix; then row n, the middle row, which every row in [lo, hi) matches to
+// tol relative (per part: L, 1/U(j,j), U).  The recurrences y_j = b_j -
+// sum L(j, j-s) y_(j-s) and x_j = (y_j - sum U(j, j+s) x_(j+s)) / U(j, j)
+// forget their state geometrically; w (a multiple of 16, at most wmax) is a
+// distance after which a unit state has decayed below tol_w in both, so a
+// solve started w entries away from zero state matches the exact one to
+// about tol_w relative.  Returns an error message for a non-positive pivot
+// or when no such w exists.
+std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, std::vector<float>& tab, int& w,
+                          int& lo, int& hi);
+
 // FindSpan (A2.1, src/basis.F90:226-259): the knot span of u, with
 // u >= U(n+1) -> n and u <= U(p) -> p.
 int find_span(int n, int p, double u, const double* U);

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace ads {
 
 namespace {
@@ -516,6 +518,243 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Segmented FP32 solve of the refinement forms: out = U^-1 L^-1 in along one
+// direction, out of place, each line cut into segments of `seg` outputs (at
+// most kSegMax), one lane per segment.  The lane runs the forward recurrence
+//   y_j = b_j - sum_s L(j, j-s) y_(j-s)
+// from W entries before its segment (zero state) to W entries after it,
+// keeping y of the segment and of the W entries after it in registers, then
+// the backward recurrence
+//   x_j = (y_j - sum_s U(j, j+s) x_(j+s)) / U(j, j)
+// from the last of those (zero state) down to the segment's first entry.
+// Both recurrences forget their state geometrically (the factors' inverses
+// decay like M^-1), so with W past that decay the result matches the
+// sequential solve to the decay tolerance; the refinement passes take that
+// up together with the FP32 rounding.  A lane then does about
+// (3 seg + 3 W) / seg steps per output with nothing sequential across lanes,
+// where the in-place sweep has one lane per line.  Row j's factor entries are
+// read from the table (zero padded beyond the line), except for blocks inside
+// [lo, hi), whose rows all equal the middle row kept in registers.
+// ---------------------------------------------------------------------------
+constexpr int kSegBlock = 16;
+static_assert(kSegMax % kSegBlock == 0, "segments of whole blocks");
+
+template <int KL, int W, class ISSUE, class TAKE, class PUT, class FLUSH>
+__device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, TAKE take, PUT put, FLUSH flush)
+{
+    constexpr int B = kSegBlock, NT = 2 * KL + 1, NY = kSegMax + W;
+    constexpr int NBF = (kSegMax + 2 * W) / B, NBB = NY / B;
+    static_assert(W % B == 0 && W <= kSegPadLo && kSegMax + W <= kSegPadHi, "warm-up inside the table padding");
+    float cst[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) cst[t] = sl.mid[t];
+    float y[NY];  // y_(ks + q)
+    float h[KL];  // h[s - 1] = y_(j - s)
+#pragma unroll
+    for (int t = 0; t < KL; ++t) h[t] = 0.0f;
+    auto fstep = [&](float in, const float* __restrict__ c) {
+        float v = in;
+#pragma unroll
+        for (int t = KL - 1; t >= 0; --t) v = fmaf(-c[t], h[t], v);
+#pragma unroll
+        for (int t = KL - 1; t > 0; --t) h[t] = h[t - 1];
+        h[0] = v;
+        return v;
+    };
+    float cur[B], raw[B];
+    issue(ks - W, raw);
+    take(raw, cur);
+#pragma unroll
+    for (int f = 0; f < NBF; ++f) {
+        const int j0 = ks - W + f * B;
+        if (f + 1 < NBF) issue(j0 + B, raw);
+        if (j0 >= sl.lo && j0 + B <= sl.hi) {
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                const float v = fstep(cur[i], cst);
+                if (f * B + i >= W) y[f * B + i - W] = v;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                const float v = fstep(cur[i], sl.rows + static_cast<long>(j0 + i) * NT);
+                if (f * B + i >= W) y[f * B + i - W] = v;
+            }
+        }
+        if (f + 1 < NBF) take(raw, cur);
+    }
+    float g[KL];  // g[s - 1] = x_(j + s)
+#pragma unroll
+    for (int t = 0; t < KL; ++t) g[t] = 0.0f;
+    auto bstep = [&](float v, const float* __restrict__ c) {
+#pragma unroll
+        for (int t = KL; t >= 1; --t) v = fmaf(-c[KL + t], g[t - 1], v);
+        const float x = v * c[KL];
+#pragma unroll
+        for (int t = KL - 1; t > 0; --t) g[t] = g[t - 1];
+        g[0] = x;
+        return x;
+    };
+#pragma unroll
+    for (int b = NBB - 1; b >= 0; --b) {
+        const int j0 = ks + b * B;
+        if (j0 >= sl.lo && j0 + B <= sl.hi) {
+#pragma unroll
+            for (int i = B - 1; i >= 0; --i) {
+                const float x = bstep(y[b * B + i], cst);
+                if (b * B < kSegMax) put(i, j0 + i, x);
+            }
+        } else {
+#pragma unroll
+            for (int i = B - 1; i >= 0; --i) {
+                const float x = bstep(y[b * B + i], sl.rows + static_cast<long>(j0 + i) * NT);
+                if (b * B < kSegMax) put(i, j0 + i, x);
+            }
+        }
+        if (b * B < kSegMax) flush(j0);
+    }
+}
+
+// Lines along a strided axis (y, z): lane = (line, segment), the line's
+// element k at base + k*stride_k, coalesced across the lanes of a wave.  The
+// result goes to out, or (fo.x set) to fo.x in FP64: x = v, x = fo.add + v
+// or x += v.
+template <int KL, int W>
+__global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in, float* __restrict__ out, int ni,
+                                                    long stride_j, long stride_k, int nlines, SegLU sl, SegOut fo)
+{
+    constexpr int B = kSegBlock;
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= nlines) return;
+    const int n = sl.n, ks = blockIdx.y * sl.seg, ke = min(ks + sl.seg, n);
+    const int i = L % ni, j = L / ni;
+    const long base = i + static_cast<long>(j) * stride_j;
+    auto issue = [&](int k0, float* v) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int k = k0 + b;
+            v[b] = (k >= 0 && k < n) ? in[base + static_cast<long>(k) * stride_k] : 0.0f;
+        }
+    };
+    auto take = [&](const float* v, float* dst) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) dst[b] = v[b];
+    };
+    auto put = [&](int, int k, float x) {
+        if (k < ke) {
+            const long at = base + static_cast<long>(k) * stride_k;
+            if (!fo.x) {
+                out[at] = x;
+            } else {
+                double v = static_cast<double>(x);
+                if (fo.mode == kSegAdd) v = static_cast<double>(fo.add[at]) + v;
+                else if (fo.mode == kSegAccumulate) v = fo.x[at] + v;
+                fo.x[at] = v;
+            }
+        }
+    };
+    seg_solve<KL, W>(ks, sl, issue, take, put, [](int) {});
+}
+
+// Contiguous rows (x): a wave owns 64 rows (one lane each) of one segment
+// and moves them through an LDS tile of kSegBlock columns, coalesced in and
+// out; the next input tile is fetched into registers while the current
+// block is computed.
+template <int KL, int W>
+__global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const float* __restrict__ in,
+                                                               float* __restrict__ out, int nrows, SegLU sl)
+{
+    constexpr int B = kSegBlock, LDT = B + 1, RPI = kWave / B, NIT = kWave / RPI;
+    static_assert(NIT == B, "one tile row per block entry");
+    __shared__ float tile[kRowWaves][kWave][LDT];
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+    const int row0 = (blockIdx.x * kRowWaves + wv) * kWave;
+    if (row0 >= nrows) return;
+    const int n = sl.n, ks = blockIdx.y * sl.seg, ke = min(ks + sl.seg, n);
+    const int cidx = lane % B, rsub = lane / B;
+    auto issue = [&](int k0, float* v) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int r = row0 + it * RPI + rsub, col = k0 + cidx;
+            v[it] = (r < nrows && col >= 0 && col < n) ? in[static_cast<long>(r) * n + col] : 0.0f;
+        }
+    };
+    auto take = [&](const float* v, float* dst) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) tile[wv][it * RPI + rsub][cidx] = v[it];
+        wave_lds_sync();
+#pragma unroll
+        for (int b = 0; b < B; ++b) dst[b] = tile[wv][lane][b];
+        wave_lds_sync();
+    };
+    auto put = [&](int i, int, float x) { tile[wv][lane][i] = x; };
+    auto flush = [&](int k0) {
+        wave_lds_sync();
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int r = row0 + it * RPI + rsub, col = k0 + cidx;
+            if (r < nrows && col < ke) out[static_cast<long>(r) * n + col] = tile[wv][it * RPI + rsub][cidx];
+        }
+        wave_lds_sync();
+    };
+    seg_solve<KL, W>(ks, sl, issue, take, put, flush);
+}
+
+// segments of at most kSegMax outputs, as even as whole lines allow
+int seg_len(int n)
+{
+    const int nseg = (n + kSegMax - 1) / kSegMax;
+    return (n + nseg - 1) / nseg;
+}
+
+template <int KL, int W>
+void seg_launch_dir(int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
+{
+    const int nx = sl[0].n, ny = sl[1].n, nz = sl[2].n;
+    const long sxy = static_cast<long>(nx) * ny;
+    SegLU v = sl[d];
+    v.seg = seg_len(v.n);
+    const int nseg = (v.n + v.seg - 1) / v.seg;
+    if (d == 0) {
+        const int rows = ny * nz, per = kWave * kRowWaves;
+        hipLaunchKernelGGL((k_seg_rows<KL, W>), dim3((rows + per - 1) / per, nseg), dim3(per), 0, s, in, out, rows,
+                           v);
+    } else if (d == 1) {
+        const int lines = nx * nz;
+        hipLaunchKernelGGL((k_seg_lines<KL, W>), dim3((lines + 255) / 256, nseg), dim3(256), 0, s, in, out, nx, sxy,
+                           static_cast<long>(nx), lines, v, SegOut{});
+    } else {
+        const int lines = nx * ny;
+        hipLaunchKernelGGL((k_seg_lines<KL, W>), dim3((lines + 255) / 256, nseg), dim3(256), 0, s, in, out, lines,
+                           0L, sxy, lines, v, fo);
+    }
+}
+
+template <int KL>
+hipError_t seg_dir_p(int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
+{
+    switch (sl[d].w) {
+    case 16: seg_launch_dir<KL, 16>(d, in, out, sl, fo, s); break;
+    case 32: seg_launch_dir<KL, 32>(d, in, out, sl, fo, s); break;
+    case 48: seg_launch_dir<KL, 48>(d, in, out, sl, fo, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t seg_dir(int p, int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
+{
+    switch (p) {
+    case 1: return seg_dir_p<1>(d, in, out, sl, fo, s);
+    case 2: return seg_dir_p<2>(d, in, out, sl, fo, s);
+    case 3: return seg_dir_p<3>(d, in, out, sl, fo, s);
+    case 4: return seg_dir_p<4>(d, in, out, sl, fo, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Kronecker residual r = b + A c - M x, A = M - dtk K (constant kappa), in
 // FP64 (c, x and b optional: nullptr means zero; with c = 0 and b = A(kappa)c
 // from the element assembly it is the mass residual b - M x of a variable
@@ -537,7 +776,9 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 //     limits, measured 187 against 212 us at 259^3);
 //   * the Z stage keeps the last 2p+1 X results of the lane in registers.
 // About 56 (70 with x) multiply-adds per DOF; HBM traffic about one read of
-// c (and x, b) and one write of r per DOF.
+// c (and x, b) and one write of r per DOF.  Without c (MO, the mass
+// residual r = b - M x) only the M products of x - 21 per DOF for p = 3 -
+// are formed.
 // Row tables hold A(i, i-KL+t), t = 0..2KL, zero outside the matrix.
 // ---------------------------------------------------------------------------
 constexpr int kResRows = 8;
@@ -589,17 +830,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 
 // four waves per SIMD (two workgroups per CU) up to p = 3; p = 4 keeps its
 // larger windows in registers at two
-template <int P, typename TX, typename TR, bool HX>
+template <int P, typename TX, typename TR, bool HX, bool MO>
 __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residual(
     const double* __restrict__ c, const TX* __restrict__ x, const double* __restrict__ b, TR* __restrict__ r,
     const double* __restrict__ mx_rows, const double* __restrict__ kx_rows, const double* __restrict__ my_rows,
     const double* __restrict__ ky_rows, const double* __restrict__ mz_rows, const double* __restrict__ kz_rows,
-    double dtk, ResDims dm, double* __restrict__ l2_part)
+    double dtk, ResDims dm, double* __restrict__ l2_part, double* __restrict__ bo)
 {
+    static_assert(!MO || HX, "the mass residual reads x");
     constexpr int KL = P - 1, BW = 2 * KL + 1;
     constexpr int OW = res_out_width(KL);
     constexpr int TYH = kResRows + 2 * KL, NTH = kWave * kResRows;
-    __shared__ double tc[2][TYH][kWave];                         // c tiles, rows y0-KL..
+    __shared__ double tc[MO ? 1 : 2][MO ? 1 : TYH][kWave];       // c tiles, rows y0-KL..
     __shared__ double td[HX ? 2 : 1][HX ? TYH : 1][kWave];       // c - x tiles
     __shared__ double sy[HX ? 1 : kResRows][2][HX ? 1 : kWave + 2 * KL];  // Y results, column lane + KL
     __shared__ double l2w[kResRows];
@@ -650,7 +892,7 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
             const int yy = y0 - KL + rr, xx = x0 + cc;
             const bool ok = i < NT && zok && yy >= 0 && yy < dm.ny && xx >= 0 && xx < dm.nx;
             const long at = xx + static_cast<long>(dm.nx) * yy + sxy * zi;
-            pc[k] = (c && ok) ? c[at] : 0.0;
+            if constexpr (!MO) pc[k] = (c && ok) ? c[at] : 0.0;
             if constexpr (HX) px[k] = ok ? x[at] : TX(0);
         }
         const int zo = zi - KL;
@@ -665,8 +907,12 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
             const int i = tid + k * NTH;
             if (i < NT) {
                 const int rr = i / kWave, cc = i - rr * kWave;
-                tc[buf][rr][cc] = pc[k];
-                if constexpr (HX) td[buf][rr][cc] = pc[k] - static_cast<double>(px[k]);
+                if constexpr (MO) {
+                    td[buf][rr][cc] = -static_cast<double>(px[k]);
+                } else {
+                    tc[buf][rr][cc] = pc[k];
+                    if constexpr (HX) td[buf][rr][cc] = pc[k] - static_cast<double>(px[k]);
+                }
             }
         }
         const double bv = pb;
@@ -676,15 +922,22 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
         double ym = 0.0, yk = 0.0, yd = 0.0;
 #pragma unroll
         for (int t = 0; t < BW; ++t) {
-            const double cv = tc[buf][wv + t][lane];
-            ym = fma(myr[t], cv, ym);
-            yk = fma(kyr[t], cv, yk);
+            if constexpr (!MO) {
+                const double cv = tc[buf][wv + t][lane];
+                ym = fma(myr[t], cv, ym);
+                yk = fma(kyr[t], cv, yk);
+            }
             if constexpr (HX) yd = fma(myr[t], td[buf][wv + t][lane], yd);
         }
-        const double c_here = tc[buf][wv + KL][lane];
+        const double c_here = MO ? 0.0 : tc[buf][wv + KL][lane];
         // X at the output columns (the halo lanes' values are not used)
         double g = 0.0, kk = 0.0, e = 0.0;
-        if constexpr (HX) {
+        if constexpr (MO) {
+            double vd[BW];
+            lane_window<KL>(yd, vd);
+#pragma unroll
+            for (int t = 0; t < BW; ++t) e = fma(mxr[t], vd[t], e);
+        } else if constexpr (HX) {
             double vm[BW], vk[BW], vd[BW];
             lane_window<KL>(ym, vm);
             lane_window<KL>(yk, vk);
@@ -708,8 +961,8 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
                 kk = fma(mxr[t], sy[wv][1][lane + t], kk);
             }
         }
-        if constexpr (HX) e = fma(-dtk, kk, e);
-        else e = fma(-dtk, kk, g);
+        if constexpr (HX && !MO) e = fma(-dtk, kk, e);
+        else if constexpr (!HX) e = fma(-dtk, kk, g);
 #pragma unroll
         for (int t = 0; t < BW - 1; ++t) {
             fw[t] = fw[t + 1];
@@ -728,11 +981,18 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
 #pragma unroll
             for (int t = 0; t < BW; ++t) {
                 o = fma(mzr[t], fw[t], o);
-                ok = fma(kzr[t], gw[t], ok);
+                if constexpr (!MO) ok = fma(kzr[t], gw[t], ok);
             }
-            o = fma(-dtk, ok, o) + bv;
-            if (out) r[gx + static_cast<long>(dm.nx) * gy + sxy * zo] = static_cast<TR>(o);
-            if (l2_part) {
+            if constexpr (!MO) o = fma(-dtk, ok, o);
+            o += bv;
+            if (out) {
+                const long at = gx + static_cast<long>(dm.nx) * gy + sxy * zo;
+                r[at] = static_cast<TR>(o);
+                if constexpr (!HX) {
+                    if (bo) bo[at] = o;
+                }
+            }
+            if (!MO && l2_part) {
                 double mc = 0.0;
 #pragma unroll
                 for (int t = 0; t < BW; ++t) mc = fma(mzr[t], gw[t], mc);
@@ -749,7 +1009,7 @@ __global__ __launch_bounds__(kWave* kResRows, P <= 4 ? 4 : 2) void k_kron_residu
         plane(zi, 0, pa, xa, ba);
         if (zi + 1 < zend) plane(zi + 1, 1, pa, xa, ba);
     }
-    if (l2_part) {
+    if (!MO && l2_part) {
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) l2 += __shfl_xor(l2, off, kWave);
         if (lane == 0) l2w[wv] = l2;
@@ -834,27 +1094,30 @@ ResDims residual_dims(const BandSet& bs, int ow, int kl, long resident)
     return dm;
 }
 
-template <int P, typename TX, typename TR, bool HX>
+template <int P, typename TX, typename TR, bool HX, bool MO>
 hipError_t residual_launch(const double* c, const TX* x, const double* b, TR* r, const BandSet& bs, double dtk,
-                           double* l2_part, int part_cap, int* nparts, hipStream_t s)
+                           double* l2_part, int part_cap, int* nparts, double* bo, hipStream_t s)
 {
     static const int resident =
-        resident_groups(reinterpret_cast<const void*>(&k_kron_residual<P, TX, TR, HX>), kWave * kResRows, 0);
+        resident_groups(reinterpret_cast<const void*>(&k_kron_residual<P, TX, TR, HX, MO>), kWave * kResRows, 0);
     const ResDims dm = residual_dims(bs, res_out_width(P - 1), P - 1, resident);
     const int blocks = dm.xchunks * dm.yblocks * ((dm.nz + dm.zseg - 1) / dm.zseg);
     *nparts = blocks;
     if (l2_part && blocks > part_cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_kron_residual<P, TX, TR, HX>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, b, r,
-                       bs.mass[0], bs.stiff[0], bs.mass[1], bs.stiff[1], bs.mass[2], bs.stiff[2], dtk, dm, l2_part);
+    hipLaunchKernelGGL((k_kron_residual<P, TX, TR, HX, MO>), dim3(blocks), dim3(kWave * kResRows), 0, s, c, x, b,
+                       r, bs.mass[0], bs.stiff[0], bs.mass[1], bs.stiff[1], bs.mass[2], bs.stiff[2], dtk, dm, l2_part,
+                       bo);
     return hipGetLastError();
 }
 
 template <int P, typename TX, typename TR>
 hipError_t residual_p(const double* c, const TX* x, const double* b, TR* r, const BandSet& bs, double dtk,
-                      double* l2_part, int part_cap, int* nparts, hipStream_t s)
+                      double* l2_part, int part_cap, int* nparts, double* bo, hipStream_t s)
 {
-    if (x) return residual_launch<P, TX, TR, true>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
-    return residual_launch<P, TX, TR, false>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
+    if (x && bo) return hipErrorInvalidValue;  // the FP64 copy is of the x = 0 residual
+    if (x && !c) return residual_launch<P, TX, TR, true, true>(c, x, b, r, bs, dtk, nullptr, part_cap, nparts, bo, s);
+    if (x) return residual_launch<P, TX, TR, true, false>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
+    return residual_launch<P, TX, TR, false, false>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
 }
 
 }  // namespace
@@ -869,6 +1132,17 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
     case 4: return kron_solve_p<T, 4>(F, sw, trans, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_seg_solve(int p, const float* in, float* tmp, float* out, const SegLU sl[3], const SegOut& fo,
+                            hipStream_t s)
+{
+    // x: in -> out, y: out -> tmp, z: tmp -> out (or fo)
+    const SegOut none{};
+    hipError_t e = seg_dir(p, 0, in, out, sl, none, s);
+    if (e == hipSuccess) e = seg_dir(p, 1, out, tmp, sl, none, s);
+    if (e == hipSuccess) e = seg_dir(p, 2, tmp, out, sl, fo, s);
+    return e;
 }
 
 template hipError_t launch_kron_solve<double>(int, double*, const SweepTables<double>[3], bool, hipStream_t);
@@ -894,25 +1168,25 @@ int resident_groups(const void* fn, int threads, size_t lds)
 
 template <typename TX, typename TR>
 hipError_t launch_kron_residual(int p, const double* c, const TX* x, const double* b, TR* r, const BandSet& bs,
-                                double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s)
+                                double dtk, double* l2_part, int part_cap, int* nparts, hipStream_t s, double* bo)
 {
     switch (p) {
-    case 1: return residual_p<2, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
-    case 2: return residual_p<3, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
-    case 3: return residual_p<4, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
-    case 4: return residual_p<5, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, s);
+    case 1: return residual_p<2, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
+    case 2: return residual_p<3, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
+    case 3: return residual_p<4, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
+    case 4: return residual_p<5, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 template hipError_t launch_kron_residual<float, float>(int, const double*, const float*, const double*, float*,
-                                                       const BandSet&, double, double*, int, int*, hipStream_t);
+                                                       const BandSet&, double, double*, int, int*, hipStream_t, double*);
 template hipError_t launch_kron_residual<double, float>(int, const double*, const double*, const double*, float*,
-                                                        const BandSet&, double, double*, int, int*, hipStream_t);
+                                                        const BandSet&, double, double*, int, int*, hipStream_t, double*);
 template hipError_t launch_kron_residual<double, double>(int, const double*, const double*, const double*, double*,
-                                                         const BandSet&, double, double*, int, int*, hipStream_t);
+                                                         const BandSet&, double, double*, int, int*, hipStream_t, double*);
 template hipError_t launch_kron_residual<float, double>(int, const double*, const float*, const double*, double*,
-                                                        const BandSet&, double, double*, int, int*, hipStream_t);
+                                                        const BandSet&, double, double*, int, int*, hipStream_t, double*);
 
 __global__ __launch_bounds__(256) void k_convert_d2f(const double* __restrict__ in, float* __restrict__ out, long n)
 {

@@ -13,6 +13,9 @@ runs on, bracketed by device synchronisation).  Alongside it:
     same state, its speed-up and its error against the FP64 step;
   * ``fp32_ir_kron``: the precision form without element assembly (two passes
     of FP64 Kronecker residual + FP32 sweeps from a zero start; constant kappa);
+  * ``fp32_ir_seg``: as ``fp32_ir_kron`` with each FP32 solve cut into line
+    segments of at most 48 entries solved in parallel, each from zero state
+    w entries outside it (the LU recurrences forget their state);
   * ``fp64_kron``: FP64 throughout, the right-hand side from the fused
     Kronecker operator of the 1-D matrices instead of the element assembly
     (exact for a constant kappa, where the Gauss sum over elements and points
@@ -207,6 +210,9 @@ def main():
              "FP32 element assembly + FP32 sweeps, then 1 FP64 Kronecker-residual refinement"),
             ("fp32_ir_kron", _lib.ADS_PREC_FP32_IR_KRON, 2,
              "zero start, 2 passes of fused FP64 Kronecker residual + FP32 sweeps (no element assembly)"),
+            ("fp32_ir_seg", _lib.ADS_PREC_FP32_IR_SEG, 2,
+             "zero start, 2 passes of fused FP64 Kronecker residual + FP32 segmented solve (lines cut into "
+             "segments of <= 48 solved in parallel from zero state 32 entries outside, truncation 1e-8)"),
             ("fp64_kron", _lib.ADS_PREC_FP64_KRON, 0,
              "FP64: fused Kronecker right-hand side (1-D Gauss matrices, constant kappa) + FP64 sweeps")):
         _lib.check(L.ads_step_d(h, vp(cdev), vp(got), None, dt, prec, refine, None), h)
@@ -341,6 +347,7 @@ def main():
         "l2norm": l2.value,
         "fp32_ir": forms["fp32_ir"],
         "fp32_ir_kron": forms["fp32_ir_kron"],
+        "fp32_ir_seg": forms["fp32_ir_seg"],
         "fp64_kron": forms["fp64_kron"],
         "grad_fwd": grad,
         "oil": oil,

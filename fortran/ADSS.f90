@@ -22,7 +22,7 @@ module ADSS
    public :: RegisterHeatForm, RegisterProjectionForm, RegisterOilForm, SyncSolution, SetDiffusivity
    public :: ads_sync_every_step, oil_drained
    public :: ADS_F0_CONST, ADS_F0_SINPROD, ADS_F0_BUMP3D, ADS_PREC_FP64, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_KRON, &
-             ADS_PREC_FP64_KRON
+             ADS_PREC_FP64_KRON, ADS_PREC_FP32_IR_SEG
 
    integer, parameter :: FORM_HEAT = 1, FORM_PROJECTION = 2, FORM_OIL = 3, MAXFORMS = 16
 

@@ -7,7 +7,7 @@ module ads_capi
       ADS_ERR_SINGULAR = 4, ADS_ERR_UNSUPPORTED = 5, ADS_ERR_NUMERIC = 6
    integer(c_int), parameter :: ADS_F0_CONST = 0, ADS_F0_SINPROD = 1, ADS_F0_BUMP3D = 2
    integer(c_int), parameter :: ADS_PREC_FP64 = 0, ADS_PREC_FP32_IR = 1, ADS_PREC_FP32_IR_KRON = 2, &
-                                 ADS_PREC_FP64_KRON = 3
+                                 ADS_PREC_FP64_KRON = 3, ADS_PREC_FP32_IR_SEG = 4
 
    interface
       integer(c_int) function ads_init(n, p, ctx) bind(C, name="ads_init")

@@ -42,6 +42,7 @@ extern "C" {
 #define ADS_PREC_FP32_IR 1 /* FP32 assembly and sweeps + FP64 residual refinement */
 #define ADS_PREC_FP32_IR_KRON 2 /* FP32 sweeps on FP64 residuals from a zero start */
 #define ADS_PREC_FP64_KRON 3    /* FP64 sweeps on the fused FP64 Kronecker right-hand side */
+#define ADS_PREC_FP32_IR_SEG 4  /* FP32 segmented solves on FP64 residuals from a zero start */
 
 /* kernels for ads_time_kernel */
 #define ADS_KERNEL_HEAT_FP64 0      /* heat RHS assembly, FP64 */
@@ -113,6 +114,13 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  *   corrections, each an FP64 residual and FP32 sweeps.
  * prec = ADS_PREC_FP32_IR_KRON: refine = 1..4 FP32 sweeps in all, on FP64
  *   residuals from a zero start (the first residual is the right-hand side).
+ * prec = ADS_PREC_FP32_IR_SEG: as ADS_PREC_FP32_IR_KRON, with each FP32 solve
+ *   cut into segments of at most 48 entries per line, each solved by the LU
+ *   recurrences started w entries outside it from zero state (the
+ *   recurrences forget their state geometrically; w, at most 48, covers a
+ *   decay to 1e-8), so a line's segments run in parallel.  Its factors are
+ *   an FP64 LU of M without interchanges (M is symmetric positive definite),
+ *   rounded to FP32.  A per-element diffusivity takes the _KRON path.
  * prec = ADS_PREC_FP64_KRON: FP64 throughout; for a constant diffusivity the
  *   right-hand side is the Kronecker operator (Mz(x)My(x)Mx - dt K) c applied
  *   with the 1-D matrices (equal to the Gauss-point assembly, whose sum over
@@ -200,6 +208,14 @@ int ads_oil_step(ads_ctx* ctx, double dt, double* l2norm, double* drained);
 /* the cached kq values, Kqvals(kx, ky, kz, ex, ey, ez) order (kx fastest):
  * E (p+1)^3 doubles */
 int ads_oil_get_kq(ads_ctx* ctx, double* kq_host);
+/* Host only: the factor tables of the segmented form for one direction of
+ * SIZE n, degree p (no reference counterpart): tab = (n+2) x (2p+1) FP32 rows
+ * (n + 1 basis functions), row j = L(j, j-1..j-p), 1/U(j, j), U(j, j+1..j+p)
+ * of the mass matrix's LU factors (zero outside the matrix), then the middle
+ * row, which rows [*lo, *hi) match to 1e-7 (L U = M, no interchanges);
+ * *w = the warm-up distance. */
+int ads_host_seg_tables(int n, int p, int* w, int* lo, int* hi, float* tab);
+
 /* curves by InitInputData's random walk with uniforms from SplitMix64(seed)
  * (the reference's own stream is unseeded): cN*cL points each of cx, cy, cz */
 int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz);

@@ -3,6 +3,7 @@ declares, and its host-side tables and generators equal the oracle's."""
 import ctypes
 
 import numpy as np
+from scipy.linalg import solve_triangular
 import pytest
 
 from ads_mpi_amd import _lib, adss
@@ -67,3 +68,102 @@ def test_oil_curves_bitwise_equal_oracle(orc, L):
         rhs = adss.OilRHS.with_random_curves(1e-4, seed, cN=cN, cL=cL)
         assert np.array_equal(rhs.cx, a.cx) and np.array_equal(rhs.cy, a.cy) and np.array_equal(rhs.cz, a.cz)
     assert L.ads_oil_curves(1, 0, 4, None, None, None) == _lib.ADS_ERR_INVALID
+
+
+SEG_MAX, SEG_BLOCK = 48, 16  # outputs per lane at most, block of the recurrences (k_seg_lines)
+
+
+def seg_emulate(rows, mid, lo, hi, w, p, b):
+    """float64 restatement of the segmented solve of the FP32 refinement form
+    (k_seg_lines / k_seg_rows): each segment of at most SEG_MAX outputs runs
+    L's recurrence from w entries before it and U's from w entries after it,
+    both from zero state; blocks of SEG_BLOCK rows inside [lo, hi) use the
+    middle row."""
+    N = len(b)
+    nseg = -(-N // SEG_MAX)
+    seg = -(-N // nseg)
+    x = np.zeros(N)
+    B = SEG_BLOCK
+
+    def coef(j0, j):
+        if j0 >= lo and j0 + B <= hi:
+            return mid
+        return rows[j] if 0 <= j < N else np.zeros(2 * p + 1)
+
+    for s in range(nseg):
+        ks, ke = s * seg, min(s * seg + seg, N)
+        y = np.zeros(SEG_MAX + w)
+        h = [0.0] * p  # h[t] = y(j-1-t)
+        for f in range((SEG_MAX + 2 * w) // B):
+            j0 = ks - w + f * B
+            for i in range(B):
+                j = j0 + i
+                c = coef(j0, j)
+                v = (b[j] if 0 <= j < N else 0.0) - sum(c[t] * h[t] for t in range(p))
+                h = [v] + h[:-1]
+                if f * B + i >= w:
+                    y[f * B + i - w] = v
+        g = [0.0] * p  # g[t-1] = x(j+t)
+        for bb in reversed(range((SEG_MAX + w) // B)):
+            j0 = ks + bb * B
+            for i in reversed(range(B)):
+                j = j0 + i
+                c = coef(j0, j)
+                v = y[bb * B + i] - sum(c[p + t] * g[t - 1] for t in range(1, p + 1))
+                xj = v * c[p]
+                g = [xj] + g[:-1]
+                if bb * B < SEG_MAX and j < ke:
+                    x[j] = xj
+    return x
+
+
+@pytest.mark.parametrize("n,p", [(258, 3), (128, 2), (20, 4), (9, 1), (385, 3), (150, 4), (200, 1)])
+def test_seg_tables_and_truncated_solve(orc, L, n, p):
+    """The segmented FP32 form's tables and algorithm, against the oracle's
+    (independently assembled) mass matrix M:
+      * the rows are LU factors of M without interchanges: L U = M to FP32
+        rounding;
+      * the warm-up w fits the decay of the factors' recurrences (about 0.27,
+        0.43, 0.54, 0.61 per entry for p = 1..4), and the interior range that
+        uses the middle row covers all but the rows near the ends;
+      * the segmented solve (restated in float64 with the FP32 tables)
+        matches the sequential solve with the same factors to the 1e-8 decay
+        tolerance, and the dense solve of M to the FP32 rounding of the
+        factors, for smooth and random right-hand sides."""
+    N = n + 1
+    nt = 2 * p + 1
+    tab = np.zeros((N + 1) * nt, dtype=np.float32)
+    w, lo, hi = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    assert L.ads_host_seg_tables(n, p, ctypes.byref(w), ctypes.byref(lo), ctypes.byref(hi),
+                                 tab.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    w, lo, hi = w.value, lo.value, hi.value
+    t = tab.reshape(N + 1, nt).astype(np.float64)
+    rows, mid = t[:N], t[N]
+    U = orc.knots(n, p)
+    M = orc.band_to_dense(orc.mkbbt_band(U, p, n, n - p + 1, (1, 0, 0, 0)), p, p)
+    Lm, Um = np.eye(N), np.zeros((N, N))
+    for j in range(N):
+        for s_ in range(1, p + 1):
+            if j - s_ >= 0:
+                Lm[j, j - s_] = rows[j, s_ - 1]
+            if j + s_ < N:
+                Um[j, j + s_] = rows[j, p + s_]
+        Um[j, j] = 1.0 / rows[j, p]
+    assert np.abs(Lm @ Um - M).max() <= 1e-6 * np.abs(M).max()
+    assert w % SEG_BLOCK == 0 and w <= {1: 16, 2: 32, 3: 32, 4: 48}[p]
+    assert 0 <= lo <= N // 2 < hi <= N
+    ne = n - p + 1
+    if N >= 100 and ne & (ne - 1) == 0:
+        # knots of a power-of-two element count are exact (the knot vector is
+        # built in single precision, so other counts vary the element lengths
+        # by about 1e-5 and few rows match the middle one to 1e-7)
+        assert lo <= 40 and hi >= N - 40
+    rng = np.random.default_rng(n * 10 + p)
+    xs = np.linspace(0.0, 1.0, N)
+    for b in (np.sin(3 * xs) + xs ** 2, rng.standard_normal(N)):
+        exact = np.linalg.solve(M, b)
+        # the same FP32 factors, solved sequentially from the line's start
+        seq = solve_triangular(Um, solve_triangular(Lm, b, lower=True, unit_diagonal=True))
+        got = seg_emulate(rows, mid, lo, hi, w, p, b)
+        assert np.linalg.norm(got - seq) <= 2e-8 * np.linalg.norm(seq)  # warm-up truncation, middle row
+        assert np.linalg.norm(got - exact) <= 5e-7 * np.linalg.norm(exact)  # FP32 factors

@@ -111,7 +111,7 @@ def test_projection_l2norm_and_precision_forms_256():
         torch.cuda.synchronize()
         _lib.check(L.ads_step_d(h, vp(cd), vp(ref), None, dt, _lib.ADS_PREC_FP64, 0, None), h)
         for prec, refine, tol in ((_lib.ADS_PREC_FP32_IR, 1, 1e-10), (_lib.ADS_PREC_FP32_IR_KRON, 2, 1e-10),
-                                  (_lib.ADS_PREC_FP64_KRON, 0, 1e-12)):
+                                  (_lib.ADS_PREC_FP32_IR_SEG, 2, 1e-10), (_lib.ADS_PREC_FP64_KRON, 0, 1e-12)):
             _lib.check(L.ads_step_d(h, vp(cd), vp(got), None, dt, prec, refine, None), h)
             _lib.check(L.ads_synchronize(h), h)
             err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))

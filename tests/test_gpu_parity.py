@@ -242,7 +242,7 @@ def test_fp32_refinement_form():
         rng = np.random.default_rng(2)
         c = rng.uniform(-1, 1, C.D)
         ref = C.step_d(c, dt, prec=0)
-        for prec, refine in ((1, 1), (1, 2), (2, 2), (2, 3)):
+        for prec, refine in ((1, 1), (1, 2), (2, 2), (2, 3), (4, 2), (4, 3)):
             got = C.step_d(c, dt, prec=prec, refine=refine)
             err = rel(got, ref)
             say("prec", prec, "refine", refine, "rel err", err)
@@ -279,6 +279,29 @@ def test_fp64_kron_form_vs_oracle(n, p):
             assert abs(l2 - l2o) <= TOL * l2o
     finally:
         adss.Cleanup(ads, data)
+
+
+@pytest.mark.parametrize("n,p", [((9, 11, 7), 1), ((20, 17, 25), 2), ((30, 21, 18), 4), ((66, 40, 35), 3),
+                                 ((70, 5, 66), 2), ((130, 66, 97), 3), ((100, 52, 140), 2), ((60, 131, 55), 4),
+                                 ((200, 24, 30), 1)])
+def test_segmented_form_vs_fp64(n, p):
+    """ADS_PREC_FP32_IR_SEG: FP32 segmented solves on FP64 Kronecker
+    residuals; two passes agree with the FP64 step to <= 1e-10, three to
+    about 1e-13.  Lines of up to 48 entries are one segment (exact start and
+    end); the longer ones here cut into 2..5 segments, 130 and 66 with
+    power-of-two element counts (whose interior rows use the middle row)."""
+    dt = 1e-7
+    C = Ctx(n, p)
+    try:
+        say("case segmented form", n, p)
+        c = np.random.default_rng(17).uniform(-1, 1, C.D)
+        ref = C.step_d(c, dt, prec=0)
+        for refine, tol in ((2, 1e-10), (3, 1e-12)):
+            err = rel(C.step_d(c, dt, prec=_lib.ADS_PREC_FP32_IR_SEG, refine=refine), ref)
+            say("refine", refine, "rel err", err)
+            assert err <= tol
+    finally:
+        C.close()
 
 
 def test_kron_ir_l2norm_and_state_api():
