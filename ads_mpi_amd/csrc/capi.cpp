@@ -417,6 +417,8 @@ int step_ir_seg(ads_ctx* ctx, const double* c_in, double* c_out, const double* k
     float* tmp = ctx->f32b;
     float* d1 = ctx->f32c;
     float* mid = ctx->f32d;  // the x-direction intermediate of the later solves
+    // the first residual (x = 0) is b = A c: kept in FP64 for the mass
+    // residuals b - M x of the later passes
     double* b = refine > 1 ? ctx->b64 : nullptr;
     if ((rc = residual<float>(ctx, c_in, nullptr, nullptr, r, dt, l2_d, b))) return rc;
     if (refine == 1) {

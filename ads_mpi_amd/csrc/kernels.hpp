@@ -142,7 +142,9 @@ struct SegOut {
     const float* add;
     int mode;
 };
-hipError_t launch_seg_solve(int p, const float* in, float* tmp, float* out, const SegLU sl[3], const SegOut& fo,
+// (TI = float: the one instance.)
+template <typename TI>
+hipError_t launch_seg_solve(int p, const TI* in, float* tmp, float* out, const SegLU sl[3], const SegOut& fo,
                             hipStream_t s);
 
 // upper bound of the workgroup (partial-sum) count of launch_kron_residual

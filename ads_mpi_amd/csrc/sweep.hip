@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ads {
 
@@ -540,8 +541,12 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 constexpr int kSegBlock = 16;
 static_assert(kSegMax % kSegBlock == 0, "segments of whole blocks");
 
-template <int KL, int W, class ISSUE, class TAKE, class PUT, class FLUSH>
-__device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, TAKE take, PUT put, FLUSH flush)
+// Output blocks (kSegBlock entries from j0) are handed over in descending
+// order: prep(j0, slot) a block ahead of the block's first put(slot, i,
+// j0 + i, x), flush(j0) after its last; slot alternates 0, 1.
+template <int KL, int W, class ISSUE, class TAKE, class PREP, class PUT, class FLUSH>
+__device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, TAKE take, PREP prep, PUT put,
+                                          FLUSH flush)
 {
     constexpr int B = kSegBlock, NT = 2 * KL + 1, NY = kSegMax + W;
     constexpr int NBF = (kSegMax + 2 * W) / B, NBB = NY / B;
@@ -587,6 +592,8 @@ __device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, 
     float g[KL];  // g[s - 1] = x_(j + s)
 #pragma unroll
     for (int t = 0; t < KL; ++t) g[t] = 0.0f;
+    constexpr int NO = kSegMax / B;  // output blocks
+    prep(ks + (NO - 1) * B, (NO - 1) & 1);
     auto bstep = [&](float v, const float* __restrict__ c) {
 #pragma unroll
         for (int t = KL; t >= 1; --t) v = fmaf(-c[KL + t], g[t - 1], v);
@@ -599,28 +606,29 @@ __device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, 
 #pragma unroll
     for (int b = NBB - 1; b >= 0; --b) {
         const int j0 = ks + b * B;
+        if (b < NO && b > 0) prep(j0 - B, (b - 1) & 1);
         if (j0 >= sl.lo && j0 + B <= sl.hi) {
 #pragma unroll
             for (int i = B - 1; i >= 0; --i) {
                 const float x = bstep(y[b * B + i], cst);
-                if (b * B < kSegMax) put(i, j0 + i, x);
+                if (b < NO) put(b & 1, i, j0 + i, x);
             }
         } else {
 #pragma unroll
             for (int i = B - 1; i >= 0; --i) {
                 const float x = bstep(y[b * B + i], sl.rows + static_cast<long>(j0 + i) * NT);
-                if (b * B < kSegMax) put(i, j0 + i, x);
+                if (b < NO) put(b & 1, i, j0 + i, x);
             }
         }
-        if (b * B < kSegMax) flush(j0);
+        if (b < NO) flush(j0);
     }
 }
 
 // Lines along a strided axis (y, z): lane = (line, segment), the line's
 // element k at base + k*stride_k, coalesced across the lanes of a wave.  The
-// result goes to out, or (fo.x set) to fo.x in FP64: x = v, x = fo.add + v
-// or x += v.
-template <int KL, int W>
+// result goes to out (OUT < 0) or to fo.x in FP64 by the SegMode OUT; the
+// operand of kSegAdd / kSegAccumulate is loaded a block ahead.
+template <int KL, int W, int OUT>
 __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in, float* __restrict__ out, int ni,
                                                     long stride_j, long stride_k, int nlines, SegLU sl, SegOut fo)
 {
@@ -641,29 +649,42 @@ __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in,
 #pragma unroll
         for (int b = 0; b < B; ++b) dst[b] = v[b];
     };
-    auto put = [&](int, int k, float x) {
-        if (k < ke) {
-            const long at = base + static_cast<long>(k) * stride_k;
-            if (!fo.x) {
-                out[at] = x;
-            } else {
-                double v = static_cast<double>(x);
-                if (fo.mode == kSegAdd) v = static_cast<double>(fo.add[at]) + v;
-                else if (fo.mode == kSegAccumulate) v = fo.x[at] + v;
-                fo.x[at] = v;
+    using OT = typename std::conditional<OUT == kSegAccumulate, double, float>::type;
+    constexpr bool kOperand = OUT == kSegAdd || OUT == kSegAccumulate;
+    OT opd[kOperand ? 2 : 1][kOperand ? B : 1];
+    auto prep = [&](int j0, int slot) {
+        if constexpr (kOperand) {
+            const OT* __restrict__ src = OUT == kSegAdd ? reinterpret_cast<const OT*>(fo.add)
+                                                        : reinterpret_cast<const OT*>(fo.x);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int k = j0 + b;
+                opd[slot][b] = k < ke ? src[base + static_cast<long>(k) * stride_k] : OT(0);
             }
         }
     };
-    seg_solve<KL, W>(ks, sl, issue, take, put, [](int) {});
+    auto put = [&](int slot, int i, int k, float x) {
+        if (k < ke) {
+            const long at = base + static_cast<long>(k) * stride_k;
+            if constexpr (OUT < 0) {
+                out[at] = x;
+            } else if constexpr (OUT == kSegSet) {
+                fo.x[at] = static_cast<double>(x);
+            } else {
+                fo.x[at] = static_cast<double>(opd[slot][i]) + static_cast<double>(x);
+            }
+        }
+    };
+    seg_solve<KL, W>(ks, sl, issue, take, prep, put, [](int) {});
 }
 
 // Contiguous rows (x): a wave owns 64 rows (one lane each) of one segment
 // and moves them through an LDS tile of kSegBlock columns, coalesced in and
 // out; the next input tile is fetched into registers while the current
 // block is computed.
-template <int KL, int W>
-__global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const float* __restrict__ in,
-                                                               float* __restrict__ out, int nrows, SegLU sl)
+template <int KL, int W, typename TI>
+__global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const TI* __restrict__ in, float* __restrict__ out,
+                                                               int nrows, SegLU sl)
 {
     constexpr int B = kSegBlock, LDT = B + 1, RPI = kWave / B, NIT = kWave / RPI;
     static_assert(NIT == B, "one tile row per block entry");
@@ -677,7 +698,7 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const float* __re
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int r = row0 + it * RPI + rsub, col = k0 + cidx;
-            v[it] = (r < nrows && col >= 0 && col < n) ? in[static_cast<long>(r) * n + col] : 0.0f;
+            v[it] = (r < nrows && col >= 0 && col < n) ? static_cast<float>(in[static_cast<long>(r) * n + col]) : 0.0f;
         }
     };
     auto take = [&](const float* v, float* dst) {
@@ -688,7 +709,7 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const float* __re
         for (int b = 0; b < B; ++b) dst[b] = tile[wv][lane][b];
         wave_lds_sync();
     };
-    auto put = [&](int i, int, float x) { tile[wv][lane][i] = x; };
+    auto put = [&](int, int i, int, float x) { tile[wv][lane][i] = x; };
     auto flush = [&](int k0) {
         wave_lds_sync();
 #pragma unroll
@@ -698,58 +719,74 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const float* __re
         }
         wave_lds_sync();
     };
-    seg_solve<KL, W>(ks, sl, issue, take, put, flush);
+    seg_solve<KL, W>(ks, sl, issue, take, [](int, int) {}, put, flush);
 }
 
 // segments of at most kSegMax outputs, as even as whole lines allow
-int seg_len(int n)
+SegLU seg_of(const SegLU& sl)
 {
-    const int nseg = (n + kSegMax - 1) / kSegMax;
-    return (n + nseg - 1) / nseg;
+    SegLU v = sl;
+    const int nseg = (v.n + kSegMax - 1) / kSegMax;
+    v.seg = (v.n + nseg - 1) / nseg;
+    return v;
+}
+
+template <int KL, int W, typename TI>
+void seg_rows_launch(const TI* in, float* out, const SegLU sl[3], hipStream_t s)
+{
+    const SegLU v = seg_of(sl[0]);
+    const int rows = sl[1].n * sl[2].n, per = kWave * kRowWaves;
+    hipLaunchKernelGGL((k_seg_rows<KL, W, TI>), dim3((rows + per - 1) / per, (v.n + v.seg - 1) / v.seg), dim3(per), 0,
+                       s, in, out, rows, v);
 }
 
 template <int KL, int W>
-void seg_launch_dir(int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
+void seg_lines_launch(int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
 {
-    const int nx = sl[0].n, ny = sl[1].n, nz = sl[2].n;
+    const int nx = sl[0].n, ny = sl[1].n;
     const long sxy = static_cast<long>(nx) * ny;
-    SegLU v = sl[d];
-    v.seg = seg_len(v.n);
+    const SegLU v = seg_of(sl[d]);
     const int nseg = (v.n + v.seg - 1) / v.seg;
-    if (d == 0) {
-        const int rows = ny * nz, per = kWave * kRowWaves;
-        hipLaunchKernelGGL((k_seg_rows<KL, W>), dim3((rows + per - 1) / per, nseg), dim3(per), 0, s, in, out, rows,
-                           v);
-    } else if (d == 1) {
-        const int lines = nx * nz;
-        hipLaunchKernelGGL((k_seg_lines<KL, W>), dim3((lines + 255) / 256, nseg), dim3(256), 0, s, in, out, nx, sxy,
-                           static_cast<long>(nx), lines, v, SegOut{});
+    if (d == 1) {
+        const int lines = nx * sl[2].n;
+        hipLaunchKernelGGL((k_seg_lines<KL, W, -1>), dim3((lines + 255) / 256, nseg), dim3(256), 0, s, in, out, nx,
+                           sxy, static_cast<long>(nx), lines, v, SegOut{});
     } else {
         const int lines = nx * ny;
-        hipLaunchKernelGGL((k_seg_lines<KL, W>), dim3((lines + 255) / 256, nseg), dim3(256), 0, s, in, out, lines,
-                           0L, sxy, lines, v, fo);
+        const dim3 grid((lines + 255) / 256, nseg);
+        const long sj = 0;
+        if (!fo.x)
+            hipLaunchKernelGGL((k_seg_lines<KL, W, -1>), grid, dim3(256), 0, s, in, out, lines, sj, sxy, lines, v, fo);
+        else if (fo.mode == kSegSet)
+            hipLaunchKernelGGL((k_seg_lines<KL, W, kSegSet>), grid, dim3(256), 0, s, in, out, lines, sj, sxy, lines, v,
+                               fo);
+        else if (fo.mode == kSegAdd)
+            hipLaunchKernelGGL((k_seg_lines<KL, W, kSegAdd>), grid, dim3(256), 0, s, in, out, lines, sj, sxy, lines, v,
+                               fo);
+        else
+            hipLaunchKernelGGL((k_seg_lines<KL, W, kSegAccumulate>), grid, dim3(256), 0, s, in, out, lines, sj, sxy,
+                               lines, v, fo);
     }
 }
 
-template <int KL>
-hipError_t seg_dir_p(int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
+// f(integral_constant KL, integral_constant W) for the degree and warm-up
+template <class F>
+hipError_t seg_dispatch(int p, int w, F&& f)
 {
-    switch (sl[d].w) {
-    case 16: seg_launch_dir<KL, 16>(d, in, out, sl, fo, s); break;
-    case 32: seg_launch_dir<KL, 32>(d, in, out, sl, fo, s); break;
-    case 48: seg_launch_dir<KL, 48>(d, in, out, sl, fo, s); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t seg_dir(int p, int d, const float* in, float* out, const SegLU sl[3], const SegOut& fo, hipStream_t s)
-{
+    auto by_w = [&](auto kl) {
+        switch (w) {
+        case 16: f(kl, std::integral_constant<int, 16>{}); break;
+        case 32: f(kl, std::integral_constant<int, 32>{}); break;
+        case 48: f(kl, std::integral_constant<int, 48>{}); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    };
     switch (p) {
-    case 1: return seg_dir_p<1>(d, in, out, sl, fo, s);
-    case 2: return seg_dir_p<2>(d, in, out, sl, fo, s);
-    case 3: return seg_dir_p<3>(d, in, out, sl, fo, s);
-    case 4: return seg_dir_p<4>(d, in, out, sl, fo, s);
+    case 1: return by_w(std::integral_constant<int, 1>{});
+    case 2: return by_w(std::integral_constant<int, 2>{});
+    case 3: return by_w(std::integral_constant<int, 3>{});
+    case 4: return by_w(std::integral_constant<int, 4>{});
     default: return hipErrorInvalidValue;
     }
 }
@@ -1134,16 +1171,27 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
     }
 }
 
-hipError_t launch_seg_solve(int p, const float* in, float* tmp, float* out, const SegLU sl[3], const SegOut& fo,
+template <typename TI>
+hipError_t launch_seg_solve(int p, const TI* in, float* tmp, float* out, const SegLU sl[3], const SegOut& fo,
                             hipStream_t s)
 {
     // x: in -> out, y: out -> tmp, z: tmp -> out (or fo)
-    const SegOut none{};
-    hipError_t e = seg_dir(p, 0, in, out, sl, none, s);
-    if (e == hipSuccess) e = seg_dir(p, 1, out, tmp, sl, none, s);
-    if (e == hipSuccess) e = seg_dir(p, 2, tmp, out, sl, fo, s);
+    hipError_t e = seg_dispatch(p, sl[0].w, [&](auto kl, auto w) {
+        seg_rows_launch<decltype(kl)::value, decltype(w)::value>(in, out, sl, s);
+    });
+    if (e == hipSuccess)
+        e = seg_dispatch(p, sl[1].w, [&](auto kl, auto w) {
+            seg_lines_launch<decltype(kl)::value, decltype(w)::value>(1, out, tmp, sl, SegOut{}, s);
+        });
+    if (e == hipSuccess)
+        e = seg_dispatch(p, sl[2].w, [&](auto kl, auto w) {
+            seg_lines_launch<decltype(kl)::value, decltype(w)::value>(2, tmp, out, sl, fo, s);
+        });
     return e;
 }
+
+template hipError_t launch_seg_solve<float>(int, const float*, float*, float*, const SegLU[3], const SegOut&,
+                                            hipStream_t);
 
 template hipError_t launch_kron_solve<double>(int, double*, const SweepTables<double>[3], bool, hipStream_t);
 template hipError_t launch_kron_solve<float>(int, float*, const SweepTables<float>[3], bool, hipStream_t);
