@@ -538,6 +538,15 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 // read from the table (zero padded beyond the line), except for blocks inside
 // [lo, hi), whose rows all equal the middle row kept in registers.
 // ---------------------------------------------------------------------------
+// consecutive workgroups go to the 8 XCDs round robin; give each XCD a
+// contiguous run of tiles so y-neighbours (which share halo rows) share an L2
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    constexpr int kXcds = 8;
+    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds, s = b / kXcds;
+    return k * q + min(k, r) + s;
+}
+
 constexpr int kSegBlock = 16;
 static_assert(kSegMax % kSegBlock == 0, "segments of whole blocks");
 
@@ -633,9 +642,13 @@ __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in,
                                                     long stride_j, long stride_k, int nlines, SegLU sl, SegOut fo)
 {
     constexpr int B = kSegBlock;
-    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    // work item (line group, segment), XCD-aware: the segments of a group
+    // (which read each other's inputs as warm-up) share an L2
+    const int n = sl.n, nseg = (n + sl.seg - 1) / sl.seg;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int L = (t / nseg) * blockDim.x + threadIdx.x;
     if (L >= nlines) return;
-    const int n = sl.n, ks = blockIdx.y * sl.seg, ke = min(ks + sl.seg, n);
+    const int ks = (t % nseg) * sl.seg, ke = min(ks + sl.seg, n);
     const int i = L % ni, j = L / ni;
     const long base = i + static_cast<long>(j) * stride_j;
     auto issue = [&](int k0, float* v) {
@@ -690,9 +703,11 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const TI* __restr
     static_assert(NIT == B, "one tile row per block entry");
     __shared__ float tile[kRowWaves][kWave][LDT];
     const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
-    const int row0 = (blockIdx.x * kRowWaves + wv) * kWave;
+    const int n = sl.n, nseg = (n + sl.seg - 1) / sl.seg;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);  // (row group, segment), as in k_seg_lines
+    const int row0 = ((t / nseg) * kRowWaves + wv) * kWave;
     if (row0 >= nrows) return;
-    const int n = sl.n, ks = blockIdx.y * sl.seg, ke = min(ks + sl.seg, n);
+    const int ks = (t % nseg) * sl.seg, ke = min(ks + sl.seg, n);
     const int cidx = lane % B, rsub = lane / B;
     auto issue = [&](int k0, float* v) {
 #pragma unroll
@@ -736,8 +751,8 @@ void seg_rows_launch(const TI* in, float* out, const SegLU sl[3], hipStream_t s)
 {
     const SegLU v = seg_of(sl[0]);
     const int rows = sl[1].n * sl[2].n, per = kWave * kRowWaves;
-    hipLaunchKernelGGL((k_seg_rows<KL, W, TI>), dim3((rows + per - 1) / per, (v.n + v.seg - 1) / v.seg), dim3(per), 0,
-                       s, in, out, rows, v);
+    const int groups = (rows + per - 1) / per, nseg = (v.n + v.seg - 1) / v.seg;
+    hipLaunchKernelGGL((k_seg_rows<KL, W, TI>), dim3(groups * nseg), dim3(per), 0, s, in, out, rows, v);
 }
 
 template <int KL, int W>
@@ -749,11 +764,11 @@ void seg_lines_launch(int d, const float* in, float* out, const SegLU sl[3], con
     const int nseg = (v.n + v.seg - 1) / v.seg;
     if (d == 1) {
         const int lines = nx * sl[2].n;
-        hipLaunchKernelGGL((k_seg_lines<KL, W, -1>), dim3((lines + 255) / 256, nseg), dim3(256), 0, s, in, out, nx,
+        hipLaunchKernelGGL((k_seg_lines<KL, W, -1>), dim3((lines + 255) / 256 * nseg), dim3(256), 0, s, in, out, nx,
                            sxy, static_cast<long>(nx), lines, v, SegOut{});
     } else {
         const int lines = nx * ny;
-        const dim3 grid((lines + 255) / 256, nseg);
+        const dim3 grid((lines + 255) / 256 * nseg);
         const long sj = 0;
         if (!fo.x)
             hipLaunchKernelGGL((k_seg_lines<KL, W, -1>), grid, dim3(256), 0, s, in, out, lines, sj, sxy, lines, v, fo);
@@ -856,14 +871,6 @@ struct ResDims {
     int xchunks, yblocks, zseg;
 };
 
-// consecutive workgroups go to the 8 XCDs round robin; give each XCD a
-// contiguous run of tiles so y-neighbours (which share halo rows) share an L2
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    constexpr int kXcds = 8;
-    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds, s = b / kXcds;
-    return k * q + min(k, r) + s;
-}
 
 // four waves per SIMD (two workgroups per CU) up to p = 3; p = 4 keeps its
 // larger windows in registers at two
