@@ -77,7 +77,7 @@ struct ads_ctx {
 
     std::vector<MixSet> mixsets;  // MultiStep operators factored so far
 
-    // band-inverse form: FP32 band tables of the mass-matrix inverses (built on first use)
+    // segmented form: FP32 factor tables per direction (built on first use)
     ads::SegLU seg[3]{};
     float* f32c = nullptr;
     float* f32d = nullptr;
