@@ -132,11 +132,22 @@ def test_fp32_forms_gradient_384x384x256_p3():
                                        adjoint_prec=_lib.ADS_PREC_FP32_IR_KRON, adjoint_refine=2))):
             cg = c0.clone().requires_grad_(True)
             kg = kap.clone().requires_grad_(True)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            torch.cuda.synchronize()
+            ev[0].record()
             cT = heat_rollout(cg, kg, solver, dt, steps, ck, **kw)
+            ev[1].record()
             loss = 0.5 * ((cT - c_obs) ** 2).sum()
             loss.backward()
+            ev[2].record()
             res[name] = (cT.detach(), cg.grad, kg.grad)
             torch.cuda.synchronize()
+            t_fwd, t_bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+            # forward throughput, and the HBM rate of its minimum traffic per
+            # step (read c and kappa, write c': 24 B per DOF) against 8 TB/s
+            say(name, "forward %.2f ms/step, %.3g DOF/s, %.1f GB/s (%.3f of 8 TB/s); gradient/forward %.2f"
+                % (t_fwd / steps, D * steps / (t_fwd * 1e-3), 24.0 * D * steps / (t_fwd * 1e-3) / 1e9,
+                   24.0 * D * steps / (t_fwd * 1e-3) / 8e12, (t_fwd + t_bwd) / t_fwd))
         e_state = float(torch.linalg.norm(res["fp32"][0] - res["fp64"][0]) / torch.linalg.norm(res["fp64"][0]))
         e_gc = float(torch.linalg.norm(res["fp32"][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))
         e_gk = float(torch.linalg.norm(res["fp32"][2] - res["fp64"][2]) / torch.linalg.norm(res["fp64"][2]))
