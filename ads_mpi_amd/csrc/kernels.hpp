@@ -124,7 +124,7 @@ hipError_t launch_oil(int p, bool tzero, const double* c, double* F, const Table
 // which rows [lo, hi) equal; w a multiple of 16 up to kSegPadLo.  in is left
 // as it is, tmp is scratch of the same size; the factors must have no row
 // interchanges.
-constexpr int kSegMax = 48;      // outputs per lane at most
+constexpr int kSegMax = 32;      // outputs per lane at most
 constexpr int kSegPadLo = 48;    // largest w
 constexpr int kSegPadHi = kSegMax + kSegPadLo;
 struct SegLU {

@@ -14,7 +14,7 @@ runs on, bracketed by device synchronisation).  Alongside it:
   * ``fp32_ir_kron``: the precision form without element assembly (two passes
     of FP64 Kronecker residual + FP32 sweeps from a zero start; constant kappa);
   * ``fp32_ir_seg``: as ``fp32_ir_kron`` with each FP32 solve cut into line
-    segments of at most 48 entries solved in parallel, each from zero state
+    segments of at most 32 entries solved in parallel, each from zero state
     w entries outside it (the LU recurrences forget their state);
   * ``fp64_kron``: FP64 throughout, the right-hand side from the fused
     Kronecker operator of the 1-D matrices instead of the element assembly
@@ -212,7 +212,7 @@ def main():
              "zero start, 2 passes of fused FP64 Kronecker residual + FP32 sweeps (no element assembly)"),
             ("fp32_ir_seg", _lib.ADS_PREC_FP32_IR_SEG, 2,
              "zero start, 2 passes of fused FP64 Kronecker residual + FP32 segmented solve (lines cut into "
-             "segments of <= 48 solved in parallel from zero state 32 entries outside, truncation 1e-8)"),
+             "segments of <= 32 solved in parallel from zero state 32 entries outside, truncation 1e-8)"),
             ("fp64_kron", _lib.ADS_PREC_FP64_KRON, 0,
              "FP64: fused Kronecker right-hand side (1-D Gauss matrices, constant kappa) + FP64 sweeps")):
         _lib.check(L.ads_step_d(h, vp(cdev), vp(got), None, dt, prec, refine, None), h)

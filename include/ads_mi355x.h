@@ -115,7 +115,7 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  * prec = ADS_PREC_FP32_IR_KRON: refine = 1..4 FP32 sweeps in all, on FP64
  *   residuals from a zero start (the first residual is the right-hand side).
  * prec = ADS_PREC_FP32_IR_SEG: as ADS_PREC_FP32_IR_KRON, with each FP32 solve
- *   cut into segments of at most 48 entries per line, each solved by the LU
+ *   cut into segments of at most 32 entries per line, each solved by the LU
  *   recurrences started w entries outside it from zero state (the
  *   recurrences forget their state geometrically; w, at most 48, covers a
  *   decay to 1e-8), so a line's segments run in parallel.  Its factors are

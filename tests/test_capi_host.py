@@ -70,7 +70,7 @@ def test_oil_curves_bitwise_equal_oracle(orc, L):
     assert L.ads_oil_curves(1, 0, 4, None, None, None) == _lib.ADS_ERR_INVALID
 
 
-SEG_MAX, SEG_BLOCK = 48, 16  # outputs per lane at most, block of the recurrences (k_seg_lines)
+SEG_MAX, SEG_BLOCK = 32, 16  # outputs per lane at most, block of the recurrences (k_seg_lines)
 
 
 def seg_emulate(rows, mid, lo, hi, w, p, b):

@@ -289,8 +289,8 @@ def test_segmented_form_vs_fp64(n, p):
     residuals; one pass agrees with the FP64 step to FP32 level (within 2x
     of the FP32 LU sweeps' one pass, or 1e-6), two to <= 1e-10, three and four to about 1e-13 (the last z solve writes
     x = v, x = d1 + v, x += v in FP64 for one, two and more passes).  Lines
-    of up to 48 entries are one segment (exact start and end); the longer
-    ones here cut into 2..5 segments, 130 and 66 with power-of-two element
+    of up to 32 entries are one segment (exact start and end); the longer
+    ones here cut into 2..7 segments, 130 and 66 with power-of-two element
     counts (whose interior rows use the middle row).  Also in place."""
     dt = 1e-7
     C = Ctx(n, p)
