@@ -403,15 +403,42 @@ int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, const double* 
     return ADS_OK;
 }
 
+// x = M^-1 b (M is symmetric, so also M^-T b) by a segmented FP32 solve and
+// `corrections` more with FP64 mass residuals b - M x, the last z solve of
+// each forming x in FP64.  x must not alias b.
+int mass_solve_seg(ads_ctx* ctx, const double* b, double* x, int corrections)
+{
+    int rc = ensure_ir_scratch(ctx);
+    if (rc || (rc = ensure_seg(ctx))) return rc;
+    float* r = ctx->f32a;
+    float* tmp = ctx->f32b;
+    float* d1 = ctx->f32c;
+    float* mid = ctx->f32d;
+    ADS_HIP(ctx, ads::launch_convert_d2f(b, r, ctx->D, ctx->stream));
+    if (corrections == 0) return seg_solve(ctx, r, tmp, d1, ads::SegOut{x, nullptr, ads::kSegSet});
+    if ((rc = seg_solve(ctx, r, tmp, d1))) return rc;
+    if ((rc = residual<float>(ctx, nullptr, d1, b, r, 0.0, nullptr))) return rc;
+    if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, d1, ads::kSegAdd}))) return rc;
+    for (int k = 1; k < corrections; ++k) {
+        if ((rc = residual<double>(ctx, nullptr, x, b, r, 0.0, nullptr))) return rc;
+        if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, nullptr, ads::kSegAccumulate}))) return rc;
+    }
+    return ADS_OK;
+}
+
 // The segmented form: as step_ir_kron with every FP32 solve replaced by the
-// segmented solve, whose last direction also forms x in FP64 (a per-element
-// kappa takes step_ir_kron's path).
+// segmented solve, whose last direction also forms x in FP64.  A per-element
+// kappa: b = A(kappa) c from the FP64 element assembly, then x = M^-1 b by
+// mass_solve_seg.
 int step_ir_seg(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa_d, double dt, int refine,
                 double* l2_d)
 {
-    if (kappa_d) return step_ir_kron(ctx, c_in, c_out, kappa_d, dt, refine, l2_d);
     int rc = ensure_ir_scratch(ctx);
     if (rc || (rc = ensure_seg(ctx))) return rc;
+    if (kappa_d) {
+        if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in, ctx->b64, dt, kappa_d, l2_d))) return rc;
+        return mass_solve_seg(ctx, ctx->b64, c_out, refine - 1);
+    }
     double* x = c_out;  // c_in is not read once b is formed
     float* r = ctx->f32a;
     float* tmp = ctx->f32b;
@@ -734,12 +761,15 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
                                                       dkappa_d, ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream));
         return ADS_OK;
     }
-    if (prec != ADS_PREC_FP32_IR && prec != ADS_PREC_FP32_IR_KRON)
+    if (prec != ADS_PREC_FP32_IR && prec != ADS_PREC_FP32_IR_KRON && prec != ADS_PREC_FP32_IR_SEG)
         return fail(ctx, ADS_ERR_INVALID, "unknown precision form");
     if (refine < 1 || refine > 4) return fail(ctx, ADS_ERR_INVALID, "refine must be 1..4");
     if ((rc = ensure_ir_scratch(ctx))) return rc;
-    // mu = M^-T lam_next: FP32 transposed sweeps corrected with FP64 mass residuals
-    if ((rc = mass_solve_ir(ctx, lam_next_d, ctx->mu, nullptr, refine - 1, true))) return rc;
+    // mu = M^-T lam_next: FP32 transposed sweeps (segmented solves: M is
+    // symmetric) corrected with FP64 mass residuals
+    if (prec == ADS_PREC_FP32_IR_SEG) rc = mass_solve_seg(ctx, lam_next_d, ctx->mu, refine - 1);
+    else rc = mass_solve_ir(ctx, lam_next_d, ctx->mu, nullptr, refine - 1, true);
+    if (rc) return rc;
     // lam_out = A(kappa) mu and dkappa in FP32 arithmetic (dkappa accumulates in FP64)
     if ((rc = zero(ctx, ctx->f32a, ctx->D))) return rc;
     ADS_HIP(ctx, ads::launch_heat_adjoint<float>(ctx->p, ctx->mu, c_n_d, ctx->f32a, ctx->tb32, dt, kappa_d, dkappa_d,

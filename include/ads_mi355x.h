@@ -120,7 +120,8 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  *   recurrences forget their state geometrically; w, at most 48, covers a
  *   decay to 1e-8), so a line's segments run in parallel.  Its factors are
  *   an FP64 LU of M without interchanges (M is symmetric positive definite),
- *   rounded to FP32.  A per-element diffusivity takes the _KRON path.
+ *   rounded to FP32.  With a per-element diffusivity, b = A(kappa) c comes
+ *   from the FP64 element assembly and the passes solve M x = b.
  * prec = ADS_PREC_FP64_KRON: FP64 throughout; for a constant diffusivity the
  *   right-hand side is the Kronecker operator (Mz(x)My(x)Mx - dt K) c applied
  *   with the 1-D matrices (equal to the Gauss-point assembly, whose sum over
@@ -173,7 +174,8 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
  *   dkappa[e] += -dt sum_{q in e} J W grad c_n . grad mu   (if dkappa_d != NULL).
  * prec = ADS_PREC_FP64 or ADS_PREC_FP64_KRON: all in FP64 (refine ignored).  ADS_PREC_FP32_IR or
  * ADS_PREC_FP32_IR_KRON: mu by FP32 transposed sweeps with refine-1 FP64
- * mass-residual corrections (refine = 1..4), then the assembly applied to
+ * mass-residual corrections (refine = 1..4); ADS_PREC_FP32_IR_SEG: the same
+ * with segmented solves (M is symmetric); then the assembly applied to
  * mu and the diffusivity gradient in FP32 arithmetic (dkappa accumulated in
  * FP64).  lam_out may equal lam_next, not c_n. */
 int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next_d,

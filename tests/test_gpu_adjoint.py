@@ -117,7 +117,8 @@ def test_fp32_forms_gradient_384x384x256_p3():
     """384x384x256 elements, p = 3, lognormal(0, 0.5) diffusivity from seed 5,
     40 steps, checkpoint 8, dt = 5e-9 (stable for the diffusivity's range):
     the forward in FP32 + refinement and the adjoint in FP32 with FP64
-    mass-residual corrections, against the all-FP64 rollout and gradient."""
+    mass-residual corrections (FP32 LU sweeps, then segmented solves),
+    against the all-FP64 rollout and gradient."""
     n, p, dt, steps, ck = (386, 386, 258), 3, 5e-9, 40, 8
     say("case fp32 forms gradient 384x384x256 p=3")
     solver = HeatSolver(n, p)
@@ -129,7 +130,9 @@ def test_fp32_forms_gradient_384x384x256_p3():
         res = {}
         for name, kw in (("fp64", dict(prec=_lib.ADS_PREC_FP64)),
                          ("fp32", dict(prec=_lib.ADS_PREC_FP32_IR_KRON, refine=2,
-                                       adjoint_prec=_lib.ADS_PREC_FP32_IR_KRON, adjoint_refine=2))):
+                                       adjoint_prec=_lib.ADS_PREC_FP32_IR_KRON, adjoint_refine=2)),
+                         ("fp32seg", dict(prec=_lib.ADS_PREC_FP32_IR_SEG, refine=2,
+                                          adjoint_prec=_lib.ADS_PREC_FP32_IR_SEG, adjoint_refine=2))):
             cg = c0.clone().requires_grad_(True)
             kg = kap.clone().requires_grad_(True)
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -148,15 +151,16 @@ def test_fp32_forms_gradient_384x384x256_p3():
             say(name, "forward %.2f ms/step, %.3g DOF/s, %.1f GB/s (%.3f of 8 TB/s); gradient/forward %.2f"
                 % (t_fwd / steps, D * steps / (t_fwd * 1e-3), 24.0 * D * steps / (t_fwd * 1e-3) / 1e9,
                    24.0 * D * steps / (t_fwd * 1e-3) / 8e12, (t_fwd + t_bwd) / t_fwd))
-        e_state = float(torch.linalg.norm(res["fp32"][0] - res["fp64"][0]) / torch.linalg.norm(res["fp64"][0]))
-        e_gc = float(torch.linalg.norm(res["fp32"][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))
-        e_gk = float(torch.linalg.norm(res["fp32"][2] - res["fp64"][2]) / torch.linalg.norm(res["fp64"][2]))
-        say("rel err: state", e_state, "grad c0", e_gc, "grad kappa", e_gk)
-        # each refined step is within 1e-10 of the FP64 step and the step is a
-        # contraction, so 40 steps stay within 40 x 1e-10
-        assert e_state <= 4e-9
-        # FP32 arithmetic in the adjoint assembly: ~1e-6 per step
-        assert e_gc <= 5e-5
-        assert e_gk <= 5e-4
+        for name in ("fp32", "fp32seg"):
+            e_state = float(torch.linalg.norm(res[name][0] - res["fp64"][0]) / torch.linalg.norm(res["fp64"][0]))
+            e_gc = float(torch.linalg.norm(res[name][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))
+            e_gk = float(torch.linalg.norm(res[name][2] - res["fp64"][2]) / torch.linalg.norm(res["fp64"][2]))
+            say(name, "rel err: state", e_state, "grad c0", e_gc, "grad kappa", e_gk)
+            # each refined step is within 1e-10 of the FP64 step and the step is a
+            # contraction, so 40 steps stay within 40 x 1e-10
+            assert e_state <= 4e-9
+            # FP32 arithmetic in the adjoint assembly: ~1e-6 per step
+            assert e_gc <= 5e-5
+            assert e_gk <= 5e-4
     finally:
         solver.close()

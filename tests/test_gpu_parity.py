@@ -385,7 +385,7 @@ def test_fp32_adjoint_against_fp64_adjoint(orc):
         kappa = adss.lognormal(5, S.E, 0.0, 0.5)
         cd, ld, kd = dev(c), dev(lam), dev(kappa)
         out = {}
-        for prec, refine in ((0, 0), (2, 2), (1, 3)):
+        for prec, refine in ((0, 0), (2, 2), (1, 3), (4, 2)):
             o = torch.empty_like(ld)
             dk = torch.zeros(S.E, dtype=torch.float64, device="cuda")
             torch.cuda.synchronize()
@@ -397,7 +397,7 @@ def test_fp32_adjoint_against_fp64_adjoint(orc):
         l64, k64 = out[(0, 0)]
         assert rel(l64, lam_o) <= TOL
         assert rel(k64, dk_o) <= 1e-11
-        for key in ((2, 2), (1, 3)):
+        for key in ((2, 2), (1, 3), (4, 2)):
             l32, k32 = out[key]
             el, ek = rel(l32, l64), rel(k32, k64)
             say("adjoint form", key, "lam rel err", el, "dkappa rel err", ek)
