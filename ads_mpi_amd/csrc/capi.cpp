@@ -330,32 +330,65 @@ int mass_solve_ir(ads_ctx* ctx, const double* b, double* x, const float* x0, int
     return ADS_OK;
 }
 
-// c_out = M^-1 A(kappa) c_in: FP32 assembly and sweeps, then `refine`
-// corrections with FP64 residuals.  Constant kappa: the residual is the
-// Kronecker operator A c - M x (it also gives l2).  Per-element kappa: b =
+// x = M^-1 b (M is symmetric, so also M^-T b) by segmented FP32 solves: the
+// first from x0 when given (an FP32 estimate, in neither f32a, f32b nor
+// f32d), else a segmented solve of b, then `corrections` more with FP64 mass
+// residuals b - M x, the last z solve of each forming x in FP64.  x must not
+// alias b.
+int mass_solve_seg(ads_ctx* ctx, const double* b, double* x, const float* x0, int corrections)
+{
+    int rc = ensure_ir_scratch(ctx);
+    if (rc || (rc = ensure_seg(ctx))) return rc;
+    float* r = ctx->f32a;
+    float* tmp = ctx->f32b;
+    float* mid = ctx->f32d;
+    const float* d1 = x0;
+    if (!d1) {
+        float* d = ctx->f32c;
+        ADS_HIP(ctx, ads::launch_convert_d2f(b, r, ctx->D, ctx->stream));
+        if (corrections == 0) return seg_solve(ctx, r, tmp, d, ads::SegOut{x, nullptr, ads::kSegSet});
+        if ((rc = seg_solve(ctx, r, tmp, d))) return rc;
+        d1 = d;
+    } else if (corrections == 0) {
+        ADS_HIP(ctx, ads::launch_add_f2d(d1, nullptr, x, ctx->D, ctx->stream));
+        return ADS_OK;
+    }
+    if ((rc = residual<float>(ctx, nullptr, d1, b, r, 0.0, nullptr))) return rc;
+    if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, d1, ads::kSegAdd}))) return rc;
+    for (int k = 1; k < corrections; ++k) {
+        if ((rc = residual<double>(ctx, nullptr, x, b, r, 0.0, nullptr))) return rc;
+        if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, nullptr, ads::kSegAccumulate}))) return rc;
+    }
+    return ADS_OK;
+}
+
+// c_out = M^-1 A(kappa) c_in: FP32 assembly and a segmented FP32 solve, then
+// `refine` corrections with FP64 residuals.  Constant kappa: the residual is
+// the Kronecker operator A c - M x (it also gives l2).  Per-element kappa: b =
 // A(kappa) c comes from the FP64 element assembly (with l2) and the
 // corrections use the mass residual b - M x.
 int step_ir(ads_ctx* ctx, const double* c_in, double* c_out, const double* kappa_d, double dt, int refine,
             double* l2_d)
 {
     int rc = ensure_ir_scratch(ctx);
-    if (rc) return rc;
-    // x accumulates in c_out unless the caller steps in place
-    double* x = c_out != c_in ? c_out : ctx->x64;
-    if ((rc = assemble<float>(ctx, ads::kFormHeat, 0, c_in, ctx->f32a, dt, kappa_d, nullptr))) return rc;
-    ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32a, ctx->sw32, false, ctx->stream));
+    if (rc || (rc = ensure_seg(ctx))) return rc;
+    float* r = ctx->f32a;
+    float* tmp = ctx->f32b;
+    float* d1 = ctx->f32c;
+    float* mid = ctx->f32d;
+    if ((rc = assemble<float>(ctx, ads::kFormHeat, 0, c_in, r, dt, kappa_d, nullptr))) return rc;
+    if ((rc = seg_solve(ctx, r, tmp, d1))) return rc;
     if (kappa_d) {
         if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in, ctx->b64, dt, kappa_d, l2_d))) return rc;
-        if ((rc = mass_solve_ir(ctx, ctx->b64, x, ctx->f32a, refine, false))) return rc;
-    } else {
-        if ((rc = residual<float>(ctx, c_in, ctx->f32a, nullptr, ctx->f32b, dt, l2_d))) return rc;
-        ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-        ADS_HIP(ctx, ads::launch_add_f2d(ctx->f32a, ctx->f32b, x, ctx->D, ctx->stream));
-        for (int k = 1; k < refine; ++k) {
-            if ((rc = residual<double>(ctx, c_in, x, nullptr, ctx->f32b, dt, nullptr))) return rc;
-            ADS_HIP(ctx, ads::launch_kron_solve<float>(ctx->p, ctx->f32b, ctx->sw32, false, ctx->stream));
-            ADS_HIP(ctx, ads::launch_add_f2d(nullptr, ctx->f32b, x, ctx->D, ctx->stream));
-        }
+        return mass_solve_seg(ctx, ctx->b64, c_out, d1, refine);
+    }
+    // the residuals read c_in: x accumulates in c_out unless the caller steps in place
+    double* x = c_out != c_in ? c_out : ctx->x64;
+    if ((rc = residual<float>(ctx, c_in, d1, nullptr, r, dt, l2_d))) return rc;
+    if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, d1, ads::kSegAdd}))) return rc;
+    for (int k = 1; k < refine; ++k) {
+        if ((rc = residual<double>(ctx, c_in, x, nullptr, r, dt, nullptr))) return rc;
+        if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, nullptr, ads::kSegAccumulate}))) return rc;
     }
     if (x != c_out)
         ADS_HIP(ctx, hipMemcpyAsync(c_out, x, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
@@ -403,29 +436,6 @@ int step_ir_kron(ads_ctx* ctx, const double* c_in, double* c_out, const double* 
     return ADS_OK;
 }
 
-// x = M^-1 b (M is symmetric, so also M^-T b) by a segmented FP32 solve and
-// `corrections` more with FP64 mass residuals b - M x, the last z solve of
-// each forming x in FP64.  x must not alias b.
-int mass_solve_seg(ads_ctx* ctx, const double* b, double* x, int corrections)
-{
-    int rc = ensure_ir_scratch(ctx);
-    if (rc || (rc = ensure_seg(ctx))) return rc;
-    float* r = ctx->f32a;
-    float* tmp = ctx->f32b;
-    float* d1 = ctx->f32c;
-    float* mid = ctx->f32d;
-    ADS_HIP(ctx, ads::launch_convert_d2f(b, r, ctx->D, ctx->stream));
-    if (corrections == 0) return seg_solve(ctx, r, tmp, d1, ads::SegOut{x, nullptr, ads::kSegSet});
-    if ((rc = seg_solve(ctx, r, tmp, d1))) return rc;
-    if ((rc = residual<float>(ctx, nullptr, d1, b, r, 0.0, nullptr))) return rc;
-    if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, d1, ads::kSegAdd}))) return rc;
-    for (int k = 1; k < corrections; ++k) {
-        if ((rc = residual<double>(ctx, nullptr, x, b, r, 0.0, nullptr))) return rc;
-        if ((rc = seg_solve(ctx, r, tmp, mid, ads::SegOut{x, nullptr, ads::kSegAccumulate}))) return rc;
-    }
-    return ADS_OK;
-}
-
 // The segmented form: as step_ir_kron with every FP32 solve replaced by the
 // segmented solve, whose last direction also forms x in FP64.  A per-element
 // kappa: b = A(kappa) c from the FP64 element assembly, then x = M^-1 b by
@@ -437,7 +447,7 @@ int step_ir_seg(ads_ctx* ctx, const double* c_in, double* c_out, const double* k
     if (rc || (rc = ensure_seg(ctx))) return rc;
     if (kappa_d) {
         if ((rc = assemble<double>(ctx, ads::kFormHeat, 0, c_in, ctx->b64, dt, kappa_d, l2_d))) return rc;
-        return mass_solve_seg(ctx, ctx->b64, c_out, refine - 1);
+        return mass_solve_seg(ctx, ctx->b64, c_out, nullptr, refine - 1);
     }
     double* x = c_out;  // c_in is not read once b is formed
     float* r = ctx->f32a;
@@ -767,7 +777,7 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
     if ((rc = ensure_ir_scratch(ctx))) return rc;
     // mu = M^-T lam_next: FP32 transposed sweeps (segmented solves: M is
     // symmetric) corrected with FP64 mass residuals
-    if (prec == ADS_PREC_FP32_IR_SEG) rc = mass_solve_seg(ctx, lam_next_d, ctx->mu, refine - 1);
+    if (prec == ADS_PREC_FP32_IR_SEG) rc = mass_solve_seg(ctx, lam_next_d, ctx->mu, nullptr, refine - 1);
     else rc = mass_solve_ir(ctx, lam_next_d, ctx->mu, nullptr, refine - 1, true);
     if (rc) return rc;
     // lam_out = A(kappa) mu and dkappa in FP32 arithmetic (dkappa accumulates in FP64)

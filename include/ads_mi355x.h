@@ -110,8 +110,9 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm);
  *   F_a = sum_e sum_q J W (v_a u - dt kappa_e grad u . grad v_a)
  *   c  <- (Mz (x) My (x) Mx)^-1 F      (Sub_Step, ADS.F90:471-794)
  * prec = ADS_PREC_FP64: FP64 throughout (refine ignored).
- * prec = ADS_PREC_FP32_IR: FP32 assembly and sweeps, then refine = 1..4
- *   corrections, each an FP64 residual and FP32 sweeps.
+ * prec = ADS_PREC_FP32_IR: FP32 assembly and an FP32 solve, then refine =
+ *   1..4 corrections, each an FP64 residual and an FP32 solve (the FP32
+ *   solves are the segmented ones of ADS_PREC_FP32_IR_SEG).
  * prec = ADS_PREC_FP32_IR_KRON: refine = 1..4 FP32 sweeps in all, on FP64
  *   residuals from a zero start (the first residual is the right-hand side).
  * prec = ADS_PREC_FP32_IR_SEG: as ADS_PREC_FP32_IR_KRON, with each FP32 solve
