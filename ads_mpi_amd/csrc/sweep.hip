@@ -551,8 +551,8 @@ constexpr int kSegBlock = 16;
 static_assert(kSegMax % kSegBlock == 0, "segments of whole blocks");
 
 // Output blocks (kSegBlock entries from j0) are handed over in descending
-// order: prep(j0, slot) a block ahead of the block's first put(slot, i,
-// j0 + i, x), flush(j0) after its last; slot alternates 0, 1.
+// order: prep(j0) before the block's recurrence steps, put(i, j0 + i, x)
+// for each, flush(j0) after its last.
 template <int KL, int W, class ISSUE, class TAKE, class PREP, class PUT, class FLUSH>
 __device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, TAKE take, PREP prep, PUT put,
                                           FLUSH flush)
@@ -602,7 +602,6 @@ __device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, 
 #pragma unroll
     for (int t = 0; t < KL; ++t) g[t] = 0.0f;
     constexpr int NO = kSegMax / B;  // output blocks
-    prep(ks + (NO - 1) * B, (NO - 1) & 1);
     auto bstep = [&](float v, const float* __restrict__ c) {
 #pragma unroll
         for (int t = KL; t >= 1; --t) v = fmaf(-c[KL + t], g[t - 1], v);
@@ -615,18 +614,18 @@ __device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, 
 #pragma unroll
     for (int b = NBB - 1; b >= 0; --b) {
         const int j0 = ks + b * B;
-        if (b < NO && b > 0) prep(j0 - B, (b - 1) & 1);
+        if (b < NO) prep(j0);
         if (j0 >= sl.lo && j0 + B <= sl.hi) {
 #pragma unroll
             for (int i = B - 1; i >= 0; --i) {
                 const float x = bstep(y[b * B + i], cst);
-                if (b < NO) put(b & 1, i, j0 + i, x);
+                if (b < NO) put(i, j0 + i, x);
             }
         } else {
 #pragma unroll
             for (int i = B - 1; i >= 0; --i) {
                 const float x = bstep(y[b * B + i], sl.rows + static_cast<long>(j0 + i) * NT);
-                if (b < NO) put(b & 1, i, j0 + i, x);
+                if (b < NO) put(i, j0 + i, x);
             }
         }
         if (b < NO) flush(j0);
@@ -636,7 +635,8 @@ __device__ __forceinline__ void seg_solve(int ks, const SegLU& sl, ISSUE issue, 
 // Lines along a strided axis (y, z): lane = (line, segment), the line's
 // element k at base + k*stride_k, coalesced across the lanes of a wave.  The
 // result goes to out (OUT < 0) or to fo.x in FP64 by the SegMode OUT; the
-// operand of kSegAdd / kSegAccumulate is loaded a block ahead.
+// operand of kSegAdd / kSegAccumulate is loaded as a block's recurrence
+// starts (a block ahead it took 16 more VGPRs and one wave per SIMD: slower).
 template <int KL, int W, int OUT>
 __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in, float* __restrict__ out, int ni,
                                                     long stride_j, long stride_k, int nlines, SegLU sl, SegOut fo)
@@ -664,19 +664,19 @@ __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in,
     };
     using OT = typename std::conditional<OUT == kSegAccumulate, double, float>::type;
     constexpr bool kOperand = OUT == kSegAdd || OUT == kSegAccumulate;
-    OT opd[kOperand ? 2 : 1][kOperand ? B : 1];
-    auto prep = [&](int j0, int slot) {
+    OT opd[kOperand ? B : 1];
+    auto prep = [&](int j0) {
         if constexpr (kOperand) {
             const OT* __restrict__ src = OUT == kSegAdd ? reinterpret_cast<const OT*>(fo.add)
                                                         : reinterpret_cast<const OT*>(fo.x);
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int k = j0 + b;
-                opd[slot][b] = k < ke ? src[base + static_cast<long>(k) * stride_k] : OT(0);
+                opd[b] = k < ke ? src[base + static_cast<long>(k) * stride_k] : OT(0);
             }
         }
     };
-    auto put = [&](int slot, int i, int k, float x) {
+    auto put = [&](int i, int k, float x) {
         if (k < ke) {
             const long at = base + static_cast<long>(k) * stride_k;
             if constexpr (OUT < 0) {
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in,
             } else if constexpr (OUT == kSegSet) {
                 fo.x[at] = static_cast<double>(x);
             } else {
-                fo.x[at] = static_cast<double>(opd[slot][i]) + static_cast<double>(x);
+                fo.x[at] = static_cast<double>(opd[i]) + static_cast<double>(x);
             }
         }
     };
@@ -695,8 +695,9 @@ __global__ __launch_bounds__(256) void k_seg_lines(const float* __restrict__ in,
 // and moves them through an LDS tile of kSegBlock columns, coalesced in and
 // out; the next input tile is fetched into registers while the current
 // block is computed.
+// five workgroups per CU (96 VGPRs) where that needs no spills: p >= 3, w <= 32
 template <int KL, int W, typename TI>
-__global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const TI* __restrict__ in, float* __restrict__ out,
+__global__ __launch_bounds__(kWave* kRowWaves, (KL >= 3 && W <= 32) ? 5 : 1) void k_seg_rows(const TI* __restrict__ in, float* __restrict__ out,
                                                                int nrows, SegLU sl)
 {
     constexpr int B = kSegBlock, LDT = B + 1, RPI = kWave / B, NIT = kWave / RPI;
@@ -724,7 +725,7 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const TI* __restr
         for (int b = 0; b < B; ++b) dst[b] = tile[wv][lane][b];
         wave_lds_sync();
     };
-    auto put = [&](int, int i, int, float x) { tile[wv][lane][i] = x; };
+    auto put = [&](int i, int, float x) { tile[wv][lane][i] = x; };
     auto flush = [&](int k0) {
         wave_lds_sync();
 #pragma unroll
@@ -734,7 +735,7 @@ __global__ __launch_bounds__(kWave* kRowWaves) void k_seg_rows(const TI* __restr
         }
         wave_lds_sync();
     };
-    seg_solve<KL, W>(ks, sl, issue, take, [](int, int) {}, put, flush);
+    seg_solve<KL, W>(ks, sl, issue, take, [](int) {}, put, flush);
 }
 
 // segments of at most kSegMax outputs, as even as whole lines allow
