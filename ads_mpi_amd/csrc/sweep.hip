@@ -519,6 +519,16 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 }
 
 // ---------------------------------------------------------------------------
+// consecutive workgroups go to the 8 XCDs round robin; give each XCD a
+// contiguous run of tiles so neighbouring tiles (which share halo data: y
+// rows of the residual, warm-up entries of the segmented solve) share an L2
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    constexpr int kXcds = 8;
+    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds, s = b / kXcds;
+    return k * q + min(k, r) + s;
+}
+
 // ---------------------------------------------------------------------------
 // Segmented FP32 solve of the refinement forms: out = U^-1 L^-1 in along one
 // direction, out of place, each line cut into segments of `seg` outputs (at
@@ -538,15 +548,6 @@ __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restric
 // read from the table (zero padded beyond the line), except for blocks inside
 // [lo, hi), whose rows all equal the middle row kept in registers.
 // ---------------------------------------------------------------------------
-// consecutive workgroups go to the 8 XCDs round robin; give each XCD a
-// contiguous run of tiles so y-neighbours (which share halo rows) share an L2
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    constexpr int kXcds = 8;
-    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds, s = b / kXcds;
-    return k * q + min(k, r) + s;
-}
-
 constexpr int kSegBlock = 16;
 static_assert(kSegMax % kSegBlock == 0, "segments of whole blocks");
 
