@@ -203,7 +203,8 @@ int ensure_seg(ads_ctx* ctx)
         if (ctx->seg[d].mid) continue;
         std::vector<float> tab;
         int w = 0, lo = 0, hi = 0;
-        const std::string e = ads::seg_lu_tables(ctx->dir[d], kSegTol, kSegTolW, ads::kSegPadLo, tab, w, lo, hi);
+        const std::string e =
+            ads::seg_lu_tables(ctx->dir[d], kSegTol, kSegTolW, ads::kSegPadLo, ads::kSegMax, tab, w, lo, hi);
         if (!e.empty()) return fail(ctx, ADS_ERR_UNSUPPORTED, "segmented form, direction " + std::to_string(d) + ": " + e);
         // device layout: the middle row, kSegPadLo zero rows, rows 0..n-1, kSegPadHi zero rows
         const int n = ctx->dir[d].dofs, nt = 2 * ctx->p + 1;
@@ -1027,7 +1028,7 @@ int ads_host_seg_tables(int n, int p, int* w, int* lo, int* hi, float* tab)
     ads::Direction d;
     if (!ads::build_direction(n, p, d).empty()) return ADS_ERR_INVALID;
     std::vector<float> t;
-    if (!ads::seg_lu_tables(d, kSegTol, kSegTolW, ads::kSegPadLo, t, *w, *lo, *hi).empty())
+    if (!ads::seg_lu_tables(d, kSegTol, kSegTolW, ads::kSegPadLo, ads::kSegMax, t, *w, *lo, *hi).empty())
         return ADS_ERR_UNSUPPORTED;
     std::copy(t.begin(), t.end(), tab);
     return ADS_OK;

@@ -310,8 +310,8 @@ void dense_matrix(const Direction& d, const double mix[4], double* out)
                 }
 }
 
-std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, std::vector<float>& tab, int& w,
-                          int& lo, int& hi)
+std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, int seg_max, std::vector<float>& tab,
+                          int& w, int& lo, int& hi)
 {
     const int N = d.dofs, p = d.p, nt = 2 * p + 1;
     // M = L U without interchanges (M is symmetric positive definite), in
@@ -378,7 +378,7 @@ std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax
         }
         need = std::max(need, last + 1);
     }
-    w = (need + 15) / 16 * 16;
+    w = N <= seg_max ? 16 : (need + 15) / 16 * 16;
     if (w > wmax) return "the factors' recurrences decay too slowly (" + std::to_string(need) + " entries)";
     return "";
 }

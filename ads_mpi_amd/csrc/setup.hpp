@@ -83,10 +83,11 @@ void sweep_tables(int N, int p, const std::vector<double>& lu, const std::vector
 // forget their state geometrically; w (a multiple of 16, at most wmax) is a
 // distance after which a unit state has decayed below tol_w in both, so a
 // solve started w entries away from zero state matches the exact one to
-// about tol_w relative.  Returns an error message for a non-positive pivot
-// or when no such w exists.
-std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, std::vector<float>& tab, int& w,
-                          int& lo, int& hi);
+// about tol_w relative.  A line of at most seg_max entries is one segment,
+// solved from its true ends (exact for any w): its w is 16.  Returns an
+// error message for a non-positive pivot or when no such w exists.
+std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, int seg_max, std::vector<float>& tab,
+                          int& w, int& lo, int& hi);
 
 // FindSpan (A2.1, src/basis.F90:226-259): the knot span of u, with
 // u >= U(n+1) -> n and u <= U(p) -> p.

@@ -117,15 +117,17 @@ def seg_emulate(rows, mid, lo, hi, w, p, b):
     return x
 
 
-@pytest.mark.parametrize("n,p", [(258, 3), (128, 2), (20, 4), (9, 1), (385, 3), (150, 4), (200, 1)])
+@pytest.mark.parametrize("n,p", [(258, 3), (128, 2), (20, 4), (9, 1), (385, 3), (150, 4), (200, 1),
+                                 (1, 1), (2, 2), (3, 3), (4, 4), (8, 4), (31, 4)])
 def test_seg_tables_and_truncated_solve(orc, L, n, p):
     """The segmented FP32 form's tables and algorithm, against the oracle's
     (independently assembled) mass matrix M:
       * the rows are LU factors of M without interchanges: L U = M to FP32
         rounding;
       * the warm-up w fits the decay of the factors' recurrences (about 0.27,
-        0.43, 0.54, 0.61 per entry for p = 1..4), and the interior range that
-        uses the middle row covers all but the rows near the ends;
+        0.43, 0.54, 0.61 per entry for p = 1..4; 16 for lines of one segment,
+        down to 2 entries), and the interior range that uses the middle row
+        covers all but the rows near the ends;
       * the segmented solve (restated in float64 with the FP32 tables)
         matches the sequential solve with the same factors to the 1e-8 decay
         tolerance, and the dense solve of M to the FP32 rounding of the
@@ -150,7 +152,8 @@ def test_seg_tables_and_truncated_solve(orc, L, n, p):
                 Um[j, j + s_] = rows[j, p + s_]
         Um[j, j] = 1.0 / rows[j, p]
     assert np.abs(Lm @ Um - M).max() <= 1e-6 * np.abs(M).max()
-    assert w % SEG_BLOCK == 0 and w <= {1: 16, 2: 32, 3: 32, 4: 48}[p]
+    # a line of at most SEG_MAX entries is one segment, exact for any w
+    assert w % SEG_BLOCK == 0 and w <= ({1: 16, 2: 32, 3: 32, 4: 48}[p] if N > SEG_MAX else 16)
     assert 0 <= lo <= N // 2 < hi <= N
     ne = n - p + 1
     if N >= 100 and ne & (ne - 1) == 0:
