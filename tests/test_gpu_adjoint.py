@@ -156,11 +156,12 @@ def test_fp32_forms_gradient_384x384x256_p3():
             e_gc = float(torch.linalg.norm(res[name][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))
             e_gk = float(torch.linalg.norm(res[name][2] - res["fp64"][2]) / torch.linalg.norm(res["fp64"][2]))
             say(name, "rel err: state", e_state, "grad c0", e_gc, "grad kappa", e_gk)
-            # each refined step is within 1e-10 of the FP64 step and the step is a
-            # contraction, so 40 steps stay within 40 x 1e-10
-            assert e_state <= 4e-9
-            # FP32 arithmetic in the adjoint assembly: ~1e-6 per step
+            # each refined step is within 1e-10 of the FP64 step and the step is
+            # non-expansive; measured 4.4e-12 after 40 steps
+            assert e_state <= 5e-11
+            # FP32 arithmetic in the adjoint assembly: ~1e-6 per step; measured
+            # 1.2e-5 (c0) and 2.7e-6 (kappa) after 40 steps
             assert e_gc <= 5e-5
-            assert e_gk <= 5e-4
+            assert e_gk <= 3e-5
     finally:
         solver.close()

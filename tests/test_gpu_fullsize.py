@@ -122,15 +122,20 @@ def test_projection_l2norm_and_precision_forms_256():
         adss.Cleanup(ads, data)
 
 
-@pytest.mark.parametrize("prec", [_lib.ADS_PREC_FP32_IR_SEG, _lib.ADS_PREC_FP32_IR_KRON])
-def test_precision_form_50_steps_256(prec):
+@pytest.mark.parametrize("prec,refine", [(_lib.ADS_PREC_FP32_IR, 1), (_lib.ADS_PREC_FP32_IR, 2),
+                                         (_lib.ADS_PREC_FP32_IR_SEG, 2), (_lib.ADS_PREC_FP32_IR_KRON, 2)])
+def test_precision_form_50_steps_256(prec, refine):
     """The precision-form run at 256^3 elements, p = 3, dt = 1e-7, 50 steps,
     u0 = projection of prod sin(pi x_i) plus uniform[-0.01, 0.01) noise from
     seed 2: the per-step error (each form step from the FP64 run's state) is
     <= 1e-10 relative L2 at every step, and the form's own 50-step trajectory
-    stays within 50 x 1e-10 of the FP64 one (max abs error printed too)."""
+    stays within 1e-10 of the FP64 one (measured ~1e-11: the step is
+    non-expansive at this dt, so the per-step errors do not grow; max abs
+    error printed too).  ADS_PREC_FP32_IR is the form as specified (FP32
+    element assembly and solve, FP64 residual corrections), with one and two
+    corrections."""
     n, p, dt, steps = 258, 3, 1e-7, 50
-    say("case 256^3 precision form, 50 steps", prec)
+    say("case 256^3 precision form, 50 steps", prec, "refine", refine)
     ads, data = adss.initialize(n, p)
     L = _lib.load()
     h = data.handle
@@ -145,8 +150,8 @@ def test_precision_form_50_steps_256(prec):
         torch.cuda.synchronize()
         for it in range(steps):
             _lib.check(L.ads_step_d(h, vp(x64), vp(nxt64), None, dt, _lib.ADS_PREC_FP64, 0, None), h)
-            _lib.check(L.ads_step_d(h, vp(x64), vp(one), None, dt, prec, 2, None), h)
-            _lib.check(L.ads_step_d(h, vp(xf), vp(xf), None, dt, prec, 2, None), h)
+            _lib.check(L.ads_step_d(h, vp(x64), vp(one), None, dt, prec, refine, None), h)
+            _lib.check(L.ads_step_d(h, vp(xf), vp(xf), None, dt, prec, refine, None), h)
             _lib.check(L.ads_synchronize(h), h)
             step_err = float(torch.linalg.norm(one - nxt64) / torch.linalg.norm(nxt64))
             worst_step = max(worst_step, step_err)
@@ -156,6 +161,6 @@ def test_precision_form_50_steps_256(prec):
         drift = float(torch.linalg.norm(xf - x64) / torch.linalg.norm(x64))
         say("worst per-step rel L2", worst_step, "worst per-step max abs", worst_max, "50-step drift", drift,
             "max abs", float(torch.max(torch.abs(xf - x64))))
-        assert drift <= steps * 1e-10
+        assert drift <= 1e-10
     finally:
         adss.Cleanup(ads, data)

@@ -163,11 +163,15 @@ def test_heat_run_16cubed_p2():
 
 
 def test_heat_steps_128cubed_p2():
-    """128^3 elements, p = 2, 130^3 DOFs, dt = 1e-6, seed 1:
-    projection + noise + 2 steps against the oracle (sum-factorised)."""
+    """The FP64 heat run at 128^3 elements, p = 2, 130^3 DOFs, dt = 1e-6,
+    seed 1 (problems/heat/main.F90:49-60: Step(0) projects u0, then steps
+    1..100): projection, noise and all 100 steps against the oracle
+    (sum-factorised, itself pinned to the faithful loops), coefficients and
+    l2norm <= 1e-12 relative at every step.  Step 1 is also checked against
+    the faithful oracle (the reference's per-point loop order)."""
     import oracle as O
 
-    n, p, dt = 129, 2, 1e-6
+    n, p, dt, steps = 129, 2, 1e-6, 100
     S = O.Setup(n, p)
     ads, data = adss.initialize(n, p)
     try:
@@ -178,13 +182,22 @@ def test_heat_steps_128cubed_p2():
         assert rel(data.F, c0) <= TOL
         c = c0 + adss.uniform(1, S.D, -0.01, 0.01)
         data.F = c
-        rhs.t = dt
-        for it in (1, 2):
-            say("case heat 128^3 p=2 step", it)
+        worst, worst_l2 = 0.0, 0.0
+        for it in range(1, steps + 1):
+            rhs.t = it * dt
             l2 = adss.Step(it, rhs, ads, data)
+            if it == 1:
+                cf, l2f = S.step(c, dt, mode=O.MODE_FAITHFUL)
             c, l2o = S.step(c, dt)
-            assert rel(data.F, c) <= TOL
-            assert abs(l2 - l2o) <= TOL * l2o
+            if it == 1:
+                assert rel(cf, c) <= 1e-13 and abs(l2f - l2o) <= 1e-13 * l2o
+            err, err_l2 = rel(data.F, c), abs(l2 - l2o) / l2o
+            worst, worst_l2 = max(worst, err), max(worst_l2, err_l2)
+            if it % 10 == 0 or err > TOL or err_l2 > TOL:
+                say("case heat 128^3 p=2 step", it, "rel err", err, "l2 rel err", err_l2)
+            assert err <= TOL, (it, err)
+            assert err_l2 <= TOL, (it, err_l2)
+        say("100 steps: worst rel err", worst, "worst l2 rel err", worst_l2)
     finally:
         adss.Cleanup(ads, data)
 
@@ -286,26 +299,29 @@ def test_fp64_kron_form_vs_oracle(n, p):
                                  ((200, 24, 30), 1), ((4, 5, 6), 4), ((2, 3, 2), 2)])
 def test_segmented_form_vs_fp64(n, p):
     """ADS_PREC_FP32_IR_SEG: FP32 segmented solves on FP64 Kronecker
-    residuals; one pass agrees with the FP64 step to FP32 level (within 4x
+    residuals; one pass agrees with the FP64 step to FP32 level (within 2x
     of the FP32 LU sweeps' one pass, or 1e-6), two to <= 1e-10, three and
-    four to <= 2e-12 (about 1e-13 but on the smallest p = 4 meshes; the
-    last z solve writes x = v, x = d1 + v, x += v in FP64 for one, two and
-    more passes).  Lines
-    of up to 32 entries are one segment (exact start and end); the longer
-    ones here cut into 2..7 segments, 130 and 66 with power-of-two element
-    counts (whose interior rows use the middle row).  Also in place."""
+    four to <= 1e-12 (the last z solve writes x = v, x = d1 + v, x += v in
+    FP64 for one, two and more passes).  Lines of up to 32 entries are one
+    segment (exact start and end); the longer ones here cut into 2..7
+    segments, 130 and 66 with power-of-two element counts (whose interior
+    rows use the middle row).  Also in place.
+
+    The two tiny meshes (n <= 6: 3 to 7 DOFs per line, p = 2 and 4) are
+    ill-conditioned enough at p = 4 that the FP32 factors of M without the
+    interchange dgbtrf makes are up to ~3x less accurate than the FP32 LU
+    sweeps (bound 4x), and passes three and four leave ~1.3e-12 there
+    (bound 2e-12)."""
     dt = 1e-7
+    tiny = max(n) <= 6
     C = Ctx(n, p)
     try:
         say("case segmented form", n, p)
         c = np.random.default_rng(17).uniform(-1, 1, C.D)
         ref = C.step_d(c, dt, prec=0)
-        # one pass: FP32 level, near the FP32 LU sweeps' one pass (the
-        # conditioning of M at p = 4 leaves ~1e-4 there; the segmented
-        # factors, without the interchange dgbtrf makes at p = 4, are up to
-        # ~3x less accurate in FP32 on the smallest meshes)
         lu1 = rel(C.step_d(c, dt, prec=_lib.ADS_PREC_FP32_IR_KRON, refine=1), ref)
-        for refine, tol in ((1, max(4 * lu1, 1e-6)), (2, 1e-10), (3, 2e-12), (4, 2e-12)):
+        k1, tight = (4, 2e-12) if tiny else (2, 1e-12)
+        for refine, tol in ((1, max(k1 * lu1, 1e-6)), (2, 1e-10), (3, tight), (4, tight)):
             err = rel(C.step_d(c, dt, prec=_lib.ADS_PREC_FP32_IR_SEG, refine=refine), ref)
             say("refine", refine, "rel err", err, "LU sweeps, one pass", lu1)
             assert err <= tol
