@@ -94,6 +94,7 @@ def load(path: str | None = None):
     L.ads_step_d.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]
     L.ads_step_adjoint_d.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.ads_eval_grid.argtypes = [_vp, I3, ctypes.c_double * 3, ctypes.c_double * 3, _dp]
+    L.ads_eval_grid_noalloc_d.argtypes = [_vp, I3, ctypes.c_double * 3, ctypes.c_double * 3, _vp]
     L.ads_host_grid_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        _ip, _dp]
     L.ads_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]

@@ -37,7 +37,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = [
-    ("setup.cpp", False),
+    ("setup.cpp", True),
     ("assemble.hip", True),
     ("sweep.hip", True),
     ("capi.cpp", True),

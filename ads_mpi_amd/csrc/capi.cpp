@@ -8,11 +8,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
 #include <vector>
 
+#include "bspline.hpp"
 #include "kernels.hpp"
 #include "setup.hpp"
 
@@ -61,6 +63,8 @@ struct ads_ctx {
 
     DevDir<double> d64[3];
     DevDir<float> d32[3];
+    double* knots[3] = {nullptr, nullptr, nullptr};  // U(0:n+p+1) per direction
+    void* grid_scratch = nullptr;  // this device's static scratch (ads_eval_grid_noalloc_d)
     double* mass[3] = {nullptr, nullptr, nullptr};
     double* stiff[3] = {nullptr, nullptr, nullptr};
     ads::Tables<double> tb64{};
@@ -504,6 +508,74 @@ int read_scalar(ads_ctx* ctx, const double* d, double* h, const char* what)
 
 }  // namespace
 
+// ---- the allocation-free grid evaluation (ads_eval_grid_noalloc_d) -------
+//
+// Its tables live in `scratch`, a static device array: per direction the
+// B-spline values [i*(p+1) + a] of up to kGridNoAllocRes grid points at
+// degree <= kGridNoAllocDeg, then per direction the first basis index of
+// each point.  Each GPU the library runs on holds one copy (540,672 bytes)
+// until the process ends, and every call on that GPU shares it: a call
+// overwrites the first res[d] entries of each table before reading them, so
+// nothing needs clearing, and calls that overlap on one GPU race.  Host code
+// reaches the array only through the address hipGetSymbolAddress returns for
+// the current device (scratch_address), never through its name.
+constexpr int kGridNoAllocRes = 4096;
+constexpr int kGridNoAllocDeg = 4;
+constexpr size_t kGridValBytes = sizeof(double) * kGridNoAllocRes * (kGridNoAllocDeg + 1);
+constexpr size_t kGridIdxBytes = sizeof(int) * kGridNoAllocRes;
+constexpr size_t kGridScratchBytes = 3 * (kGridValBytes + kGridIdxBytes);
+static_assert(kGridValBytes % 256 == 0, "tables start 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kGridScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+int scratch_address(ads_ctx* ctx, void** out)
+{
+    int dev = -1;
+    ADS_HIP(ctx, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ctx, ADS_ERR_UNSUPPORTED, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* addr = nullptr;
+        ADS_HIP(ctx, hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+        if (!addr) return fail(ctx, ADS_ERR_HIP, "hipGetSymbolAddress returned a null address for the grid scratch");
+        g_scratch_addr[dev] = addr;
+    }
+    *out = g_scratch_addr[dev];
+    return ADS_OK;
+}
+
+struct GridDir {
+    const double* U;  // knots on the device
+    int n, res;
+    double lo, hi;
+    double* vals;  // res * (p+1)
+    int* first;    // res
+};
+struct GridArgs {
+    GridDir d[3];
+    int p;
+};
+
+// One thread per (grid point, direction): the point's first basis index
+// and B-spline values, rounded exactly as the host tables (grid_point).
+__global__ __launch_bounds__(256) void k_grid_tables(GridArgs g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const GridDir& a = g.d[blockIdx.y];
+    if (i >= a.res) return;
+    ads::grid_point(a.n, g.p, a.U, a.res, a.lo, a.hi, i, a.first + i, a.vals + static_cast<long>(i) * (g.p + 1));
+}
+
+}  // namespace
+
 extern "C" {
 
 int ads_init(const int n[3], const int p[3], ads_ctx** out)
@@ -543,6 +615,8 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
         if (!rc) rc = upload(ctx, &ctx->mass[d], ctx->dir[d].mass_rows);
         if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->dir[d].stiff_rows);
     }
+    for (int d = 0; d < 3 && !rc; ++d) rc = upload(ctx, &ctx->knots[d], ctx->dir[d].knots);
+    if (!rc) rc = scratch_address(ctx, &ctx->grid_scratch);
     if (!rc) rc = dev_alloc(ctx, &ctx->c, ctx->D);
     if (!rc) rc = dev_alloc(ctx, &ctx->F, ctx->D);
     {
@@ -830,6 +904,41 @@ int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const doub
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     release();
     if (e != hipSuccess) return hip_fail(ctx, e, "ads_eval_grid");
+    return ADS_OK;
+}
+
+int ads_eval_grid_noalloc_d(ads_ctx* ctx, const int res[3], const double lo[3], const double hi[3], double* vals_d)
+{
+    if (check_ctx(ctx) || !res || !lo || !hi || !vals_d) return ADS_ERR_INVALID;
+    if (ctx->p > kGridNoAllocDeg)
+        return fail(ctx, ADS_ERR_UNSUPPORTED, "ads_eval_grid_noalloc_d: degree above 4");
+    int rmax = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (res[d] < 2 || res[d] > kGridNoAllocRes)
+            return fail(ctx, ADS_ERR_INVALID,
+                        "ads_eval_grid_noalloc_d: res must be 2..4096 per direction (the size of its static tables)");
+        if (!std::isfinite(lo[d]) || !std::isfinite(hi[d]))
+            return fail(ctx, ADS_ERR_INVALID, "ads_eval_grid_noalloc_d: bad range");
+        rmax = std::max(rmax, res[d]);
+    }
+    if (!ctx->grid_scratch) return fail(ctx, ADS_ERR_HIP, "ads_eval_grid_noalloc_d: no scratch address");
+    unsigned char* base = static_cast<unsigned char*>(ctx->grid_scratch);
+    GridArgs g{};
+    g.p = ctx->p;
+    for (int d = 0; d < 3; ++d) {
+        g.d[d].U = ctx->knots[d];
+        g.d[d].n = ctx->dir[d].n;
+        g.d[d].res = res[d];
+        g.d[d].lo = lo[d];
+        g.d[d].hi = hi[d];
+        g.d[d].vals = reinterpret_cast<double*>(base + d * kGridValBytes);
+        g.d[d].first = reinterpret_cast<int*>(base + 3 * kGridValBytes + d * kGridIdxBytes);
+    }
+    hipLaunchKernelGGL(k_grid_tables, dim3(static_cast<unsigned>((rmax + 255) / 256), 3), dim3(256), 0, ctx->stream, g);
+    ADS_HIP(ctx, hipGetLastError());
+    ADS_HIP(ctx, ads::launch_eval_grid(ctx->p, ctx->c, ctx->dir[0].dofs, ctx->dir[1].dofs, g.d[0].first, g.d[0].vals,
+                                       res[0], g.d[1].first, g.d[1].vals, res[1], g.d[2].first, g.d[2].vals, res[2],
+                                       vals_d, ctx->stream));
     return ADS_OK;
 }
 

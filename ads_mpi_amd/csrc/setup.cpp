@@ -13,6 +13,8 @@
 //                 (src/ADS.F90:352-361), including its zero-pivot failure.
 #include "setup.hpp"
 
+#include "bspline.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -20,8 +22,6 @@
 namespace ads {
 
 namespace {
-
-constexpr int kMaxDegree = 9;
 
 // Gauss-Legendre nodes and weights on [-1, 1].
 bool gauss_rule(int q, std::vector<double>& x, std::vector<double>& w)
@@ -97,41 +97,6 @@ bool gauss_rule(int q, std::vector<double>& x, std::vector<double>& w)
 
 // Values and first derivatives of the p+1 B-splines that are non-zero on
 // span i at u (A2.3 with n = 1 derivative).  out[a*2 + k].
-void basis_and_derivative(int span, double u, int p, const double* U, double* out)
-{
-    double left[kMaxDegree + 1], right[kMaxDegree + 1];
-    double ndu[kMaxDegree + 1][kMaxDegree + 1];  // ndu[j][r]: lower = knot diffs, upper = values
-    ndu[0][0] = 1.0;
-    for (int j = 1; j <= p; ++j) {
-        left[j] = u - U[span + 1 - j];
-        right[j] = U[span + j] - u;
-        double carry = 0.0;
-        for (int r = 0; r < j; ++r) {
-            ndu[j][r] = right[r + 1] + left[j - r];
-            const double t = ndu[r][j - 1] / ndu[j][r];
-            ndu[r][j] = carry + right[r + 1] * t;
-            carry = left[j - r] * t;
-        }
-        ndu[j][j] = carry;
-    }
-    for (int r = 0; r <= p; ++r) {
-        out[r * 2 + 0] = ndu[r][p];
-        // first derivative (k = 1 of A2.3)
-        double a_prev0 = 1.0, a_cur0 = 0.0, a_curk = 0.0;
-        double der = 0.0;
-        const int rk = r - 1, pk = p - 1;
-        if (r >= 1) {
-            a_cur0 = a_prev0 / ndu[pk + 1][rk];
-            der = a_cur0 * ndu[rk][pk];
-        }
-        // j loop of A2.3 is empty for k = 1 (j1 = 1 > j2 = 0 or j2 = p - r <= 0)
-        if (r <= pk) {
-            a_curk = -a_prev0 / ndu[pk + 1][r];
-            der = der + a_curk * ndu[r][pk];
-        }
-        out[r * 2 + 1] = der * p;
-    }
-}
 
 }  // namespace
 
@@ -407,19 +372,6 @@ void sweep_tables(int N, int p, const std::vector<double>& lu, const std::vector
     }
 }
 
-int find_span(int n, int p, double u, const double* U)
-{
-    if (u >= U[n + 1]) return n;
-    if (u <= U[p]) return p;
-    int low = p, high = n + 1;
-    int mid = (low + high) / 2;
-    while (u < U[mid] || u >= U[mid + 1]) {
-        if (u < U[mid]) high = mid;
-        else low = mid;
-        mid = (low + high) / 2;
-    }
-    return mid;
-}
 
 void grid_tables(const Direction& d, int res, double lo, double hi, std::vector<int>& first,
                  std::vector<double>& vals)
@@ -427,18 +379,8 @@ void grid_tables(const Direction& d, int res, double lo, double hi, std::vector<
     const int P = d.p + 1;
     first.assign(res, 0);
     vals.assign(static_cast<size_t>(res) * P, 0.0);
-    double bv[(kMaxDegree + 1) * 2];
-    for (int i = 0; i < res; ++i) {
-        const double t = i / static_cast<double>(res - 1);
-        // (1 - t) * lo + t * hi, each product rounded (no fused multiply-add)
-        const double a = (1.0 - t) * lo;
-        const double b = t * hi;
-        const double u = a + b;
-        const int span = find_span(d.n, d.p, u, d.knots.data());
-        basis_and_derivative(span, u, d.p, d.knots.data(), bv);
-        first[i] = span - d.p;
-        for (int k = 0; k < P; ++k) vals[static_cast<size_t>(i) * P + k] = bv[k * 2];
-    }
+    for (int i = 0; i < res; ++i)
+        grid_point(d.n, d.p, d.knots.data(), res, lo, hi, i, &first[i], &vals[static_cast<size_t>(i) * P]);
 }
 
 uint64_t splitmix64(uint64_t x)

@@ -89,10 +89,6 @@ void sweep_tables(int N, int p, const std::vector<double>& lu, const std::vector
 std::string seg_lu_tables(const Direction& d, double tol, double tol_w, int wmax, int seg_max, std::vector<float>& tab,
                           int& w, int& lo, int& hi);
 
-// FindSpan (A2.1, src/basis.F90:226-259): the knot span of u, with
-// u >= U(n+1) -> n and u <= U(p) -> p.
-int find_span(int n, int p, double u, const double* U);
-
 // Tables of the output path (SaveSplinePlot, src/plot.F90:97-145): for the
 // res points u_i = (1 - t) lo + t hi, t = i / (res - 1) (math.F90:44-51),
 // first[i] = span - p and vals[i*(p+1) + a] = the p+1 non-zero B-splines at

@@ -233,6 +233,18 @@ int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double
  * VtkOutput writes (src/vtk.F90:55-60).  res = 2..65536 per direction. */
 int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const double hi[3], double* vals);
 
+/* No counterpart in the reference: the output grid of ads_eval_grid for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same points and
+ * values as ads_eval_grid, written to the caller's device buffer vals_d
+ * (res_x*res_y*res_z doubles, x fastest) on the context stream, without
+ * synchronising.  It calls no hipMalloc/hipFree: its tables go to a static
+ * device array of 540,672 bytes that each GPU holds until the process ends,
+ * so res is limited to 2..4096 per direction (ADS_ERR_INVALID beyond) and
+ * the degree to 4.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads, other contexts) race. */
+int ads_eval_grid_noalloc_d(ads_ctx* ctx, const int res[3], const double lo[3], const double hi[3], double* vals_d);
+
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 
 /* The tables ads_init builds for one direction, without a device: knots U
