@@ -35,6 +35,18 @@ ADS_PREC_FP32_IR_SEG = 4
 ADS_KERNEL_HEAT_FP64 = 0
 ADS_KERNEL_KRON_RESIDUAL = 1
 ADS_KERNEL_OIL = 2
+ADS_KERNEL_SOLVE_FP64 = 3
+ADS_KERNEL_SOLVE_FP64_LU = 4
+ADS_KERNEL_SOLVE_SEG_FP32 = 5
+
+ADS_FORM_HEAT = 0
+ADS_FORM_PROJECT = 1
+ADS_FORM_OIL = 2
+ADS_FORM_OIL_PROJECT = 3
+
+ADS_SOLVER_AUTO = 0
+ADS_SOLVER_LU = 1
+ADS_SOLVER_PARTITIONED = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -108,8 +120,23 @@ def load(path: str | None = None):
     L.ads_fill_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_fill_lognormal.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.ads_host_basis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _ip, _dp, _dp, _dp, _dp]
+    L.ads_form_un.argtypes = [_vp, ctypes.c_int]
+    L.ads_form_rhs_sub.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp,
+                                   _dp]
+    L.ads_sub_step.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double * 4, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_double, _dp]
+    L.ads_last_drained.argtypes = [_vp, _dp]
+    L.ads_set_solver.argtypes = [_vp, ctypes.c_int]
+    L.ads_host_part_tables.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.ads_matrix_create.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double * 4,
+                                    ctypes.POINTER(_vp)]
+    L.ads_matrix_destroy.argtypes = [_vp]
+    L.ads_matrix_last_error.argtypes = [_vp]
+    L.ads_matrix_last_error.restype = ctypes.c_char_p
+    L.ads_matrix_dense.argtypes = [_vp, _dp]
+    L.ads_matrix_solve.argtypes = [_vp, _dp, ctypes.c_int]
     for name in header_symbols():
-        if name != "ads_last_error":
+        if name not in ("ads_last_error", "ads_matrix_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -36,8 +36,8 @@ from ._lib import (ADS_F0_BUMP3D, ADS_F0_CONST, ADS_F0_SINPROD, ADS_PREC_FP32_IR
                    ADS_PREC_FP32_IR_KRON, ADS_PREC_FP64, ADS_PREC_FP64_KRON, AdsError, check)
 
 __all__ = [
-    "ADS_setup", "ADS_compute_data", "initialize", "Step", "MultiStep", "Cleanup", "Form3DRHS", "FormUn",
-    "ComputeMatrix", "HeatRHS", "OilRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
+    "ADS_setup", "ADS_compute_data", "initialize", "Step", "MultiStep", "Sub_Step", "Cleanup", "Form3DRHS",
+    "FormUn", "ComputeMatrix", "SolveOneDirection", "SparseMatrix", "HeatRHS", "OilRHS", "L2ProjectionRHS", "AdsError", "ADS_F0_CONST", "ADS_F0_SINPROD",
     "ADS_F0_BUMP3D", "ADS_PREC_FP64", "ADS_PREC_FP32_IR", "ADS_PREC_FP32_IR_KRON", "ADS_PREC_FP64_KRON", "ADS_PREC_FP32_IR_SEG", "PlotParams", "EvalGrid", "VtkOutput",
     "PrintSolution",
 ]
@@ -306,44 +306,127 @@ def MultiStep(iter, mix, RHS_fun, ads, ads_data):
     return l2.value
 
 
-def Form3DRHS(ads, ads_data, RHS_fun):
-    """projection_engine::Form3DRHS after FormUn (projection_engine.F90:154-284)
-    for the heat integrand at t > 0.  Returns (F, l2norm) on the host without
-    changing the state."""
-    import torch  # device buffers
+def _form_of(RHS_fun, ads_data, iter=None):
+    """(form, f0 kind, dt) of a registered integrand: the heat and oil forms
+    take their t == 0 branch (the projection of their initial state) when t
+    is 0, as the reference's integrands test it."""
+    if isinstance(RHS_fun, L2ProjectionRHS):
+        return _lib.ADS_FORM_PROJECT, ADS_F0_CONST, 0.0
+    if isinstance(RHS_fun, OilRHS):
+        RHS_fun.setup(ads_data)
+        if RHS_fun.t == 0.0:
+            return _lib.ADS_FORM_OIL_PROJECT, ADS_F0_CONST, 0.0
+        return _lib.ADS_FORM_OIL, ADS_F0_CONST, RHS_fun.Dt
+    if isinstance(RHS_fun, HeatRHS):
+        _set_kappa(ads_data, RHS_fun.kappa)
+        if RHS_fun.t == 0.0:
+            return _lib.ADS_FORM_PROJECT, RHS_fun.f0, 0.0
+        return _lib.ADS_FORM_HEAT, RHS_fun.f0, RHS_fun.Dt
+    raise AdsError(_lib.ADS_ERR_UNSUPPORTED,
+                   "RHS_fun is not a registered device form (HeatRHS, OilRHS, L2ProjectionRHS)")
 
+
+def Form3DRHS(ads, ads_data, direction, substep, RHS_fun):
+    """projection_engine::Form3DRHS (projection_engine.F90:154-284) with a
+    registered integrand form, on the current coefficients (point values of
+    U_n kept by FormUn(1), gradients of the current coefficients).  Returns
+    (F, l2norm): F on the host, x fastest (the reference's ads_data%F); the
+    state is unchanged."""
     L = _lib.load()
     h = ads_data.handle
-    if not isinstance(RHS_fun, HeatRHS) or RHS_fun.t == 0.0:
-        raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "Form3DRHS: only the t > 0 heat form is exposed")
-    F = torch.empty(ads_data.D, dtype=torch.float64, device="cuda")
-    l2 = torch.empty(1, dtype=torch.float64, device="cuda")
-    kap = None
-    if RHS_fun.kappa is not None:
-        kap = torch.as_tensor(np.ascontiguousarray(RHS_fun.kappa, dtype=np.float64), device="cuda")
-    torch.cuda.synchronize()
-    check(L.ads_form_rhs_d(h, ctypes.c_void_p(ads_data.device_ptr()),
-                           ctypes.c_void_p(kap.data_ptr()) if kap is not None else None,
-                           RHS_fun.Dt, ctypes.c_void_p(F.data_ptr()), ctypes.c_void_p(l2.data_ptr())), h)
-    check(L.ads_synchronize(h), h)
-    return F.cpu().numpy(), float(l2.cpu()[0])
+    form, f0, dt = _form_of(RHS_fun, ads_data)
+    F = np.empty(ads_data.D)
+    l2 = ctypes.c_double(0.0)
+    check(L.ads_form_rhs_sub(h, form, f0, int(direction), int(substep), dt, _d(F), ctypes.byref(l2)), h)
+    return F, l2.value
 
 
 def FormUn(subun, ads, ads_data):
-    """projection_engine::FormUn (:306-441) is fused into the assembly kernel:
-    values and gradients at the Gauss points are never stored."""
-    raise AdsError(_lib.ADS_ERR_UNSUPPORTED, "FormUn is fused into Form3DRHS on the GPU")
-
-
-def ComputeMatrix(ads, ads_data, direction, mix):
-    """projection_engine::ComputeMatrix / MKBBT (:531-544, :63-132): the dense
-    1-D matrix mix(1)M + mix(2)K + mix(3)B + mix(4)B^T of one direction."""
+    """projection_engine::FormUn (:306-441).  The point values and gradients
+    are formed inside the assembly kernels: subun = 1 keeps the current
+    coefficients as U_n (the point values of the step's later right-hand
+    sides, the reference's Un); subun = 2, 3 keep U_n (the gradients always
+    come from the current coefficients)."""
     L = _lib.load()
-    N = int(ads.n[direction]) + 1
-    out = np.zeros((N, N))
-    m = np.asarray(mix, dtype=np.float64)
-    check(L.ads_get_matrix(ads_data.handle, int(direction), _d(m), _d(out)), ads_data.handle)
-    return out
+    check(L.ads_form_un(ads_data.handle, int(subun)), ads_data.handle)
+
+
+def Sub_Step(ads, iter, mix, direction, substep, RHS_fun, ads_data):
+    """ADSS::Sub_Step (src/ADS.F90:471-794): Form3DRHS with the registered
+    form, then the solves with mix(1)M + mix(2)K + mix(3)B + mix(4)B^T in x,
+    y and z; the result becomes the coefficients.  Returns l2norm."""
+    L = _lib.load()
+    h = ads_data.handle
+    form, f0, dt = _form_of(RHS_fun, ads_data)
+    m = (ctypes.c_double * 4)(*[float(v) for v in np.asarray(mix, dtype=np.float64).ravel()[:4]])
+    l2 = ctypes.c_double(0.0)
+    check(L.ads_sub_step(h, form, f0, m, int(direction), int(substep), dt, ctypes.byref(l2)), h)
+    if form == _lib.ADS_FORM_OIL:
+        dr = ctypes.c_double(0.0)
+        check(L.ads_last_drained(h, ctypes.byref(dr)), h)
+        RHS_fun.drained += dr.value
+    ads_data.t = getattr(RHS_fun, "t", 0.0)
+    return l2.value
+
+
+class SparseMatrix:
+    """The 1-D matrix of ComputeMatrix (type(sparse_matrix), src/sparse.F90)
+    with its banded LU, held by the library (ads_matrix)."""
+
+    def __init__(self, U, p, n, nelem, mix):
+        L = _lib.load()
+        self.n, self.p = int(n), int(p)
+        u = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        if u.size != self.n + self.p + 2:
+            raise ValueError("U must hold n + p + 2 knots")
+        m = (ctypes.c_double * 4)(*[float(v) for v in np.asarray(mix, dtype=np.float64).ravel()[:4]])
+        self.handle = ctypes.c_void_p()
+        rc = L.ads_matrix_create(_d(u), self.n, self.p, int(nelem), m, ctypes.byref(self.handle))
+        if rc != 0:
+            raise AdsError(rc, L.ads_matrix_last_error(None).decode())
+
+    def dense(self):
+        N = self.n + 1
+        out = np.zeros((N, N))
+        check(_lib.load().ads_matrix_dense(self.handle, _d(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            _lib.load().ads_matrix_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ComputeMatrix(U, p, n, nelem, mix):
+    """projection_engine::ComputeMatrix / MKBBT (:531-544, :63-132): the 1-D
+    matrix mix(1)M + mix(2)K + mix(3)B + mix(4)B^T of the degree-p B-splines
+    on the knot vector U(0:n+p+1) with nelem elements, factored for
+    SolveOneDirection."""
+    return SparseMatrix(U, p, n, nelem, mix)
+
+
+def SolveOneDirection(RHS, eqnum, n, p, sprsmtrx):
+    """ADSS::SolveOneDirection (src/ADS.F90:294-379): the columns
+    RHS[:, 0:eqnum] of the (n+1) x eqnum array RHS (the Fortran RHS(1:n+1,
+    1:eqnum)) are overwritten by the solutions, computed on the GPU.  Returns
+    RHS."""
+    L = _lib.load()
+    if sprsmtrx.n != int(n) or sprsmtrx.p != int(p):
+        raise ValueError("SolveOneDirection: the matrix was built for another n or p")
+    if RHS.ndim != 2 or RHS.shape[0] != int(n) + 1 or RHS.shape[1] < int(eqnum):
+        raise ValueError("SolveOneDirection: RHS must be (n+1) x eqnum")
+    buf = np.ascontiguousarray(RHS[:, :int(eqnum)].T, dtype=np.float64)  # one system per row
+    rc = L.ads_matrix_solve(sprsmtrx.handle, _d(buf), int(eqnum))
+    if rc != 0:
+        raise AdsError(rc, L.ads_matrix_last_error(sprsmtrx.handle).decode())
+    RHS[:, :int(eqnum)] = buf.T
+    return RHS
 
 
 @dataclass

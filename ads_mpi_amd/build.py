@@ -38,8 +38,10 @@ HIP_FLAGS = [
 
 SOURCES = [
     ("setup.cpp", True),
+    ("part_tables.cpp", False),
     ("assemble.hip", True),
     ("sweep.hip", True),
+    ("part.hip", True),
     ("capi.cpp", True),
 ]
 

@@ -16,6 +16,7 @@
 
 #include "bspline.hpp"
 #include "kernels.hpp"
+#include "part.hpp"
 #include "setup.hpp"
 
 namespace {
@@ -64,6 +65,17 @@ struct ads_ctx {
     DevDir<double> d64[3];
     DevDir<float> d32[3];
     double* knots[3] = {nullptr, nullptr, nullptr};  // U(0:n+p+1) per direction
+    // the partitioned FP64 solve of M (part.hpp), when every direction's
+    // lines fit its kernels; solver: ADS_SOLVER_*
+    ads::PartDev<double> part[3]{};
+    bool part_ok = false;
+    int solver = ADS_SOLVER_AUTO;
+    // FormUn: a copy of the coefficients kept by ads_form_un(1) (U_n) and the
+    // state version it was taken at (version counts the changes of c)
+    double* un = nullptr;
+    bool un_kept = false;
+    long un_version = -1;
+    long version = 0;
     void* grid_scratch = nullptr;  // this device's static scratch (ads_eval_grid_noalloc_d)
     double* mass[3] = {nullptr, nullptr, nullptr};
     double* stiff[3] = {nullptr, nullptr, nullptr};
@@ -102,6 +114,24 @@ struct ads_ctx {
     } oil;
 
     std::vector<void*> allocs;
+};
+
+// One 1-D matrix of ComputeMatrix (MKBBT on the caller's knots) with its
+// banded LU; the device copies of its sweep tables and a right-hand-side
+// buffer are made on the first solve.
+struct ads_matrix {
+    std::string err;
+    ads::Direction dir;
+    double mix[4] = {0.0, 0.0, 0.0, 0.0};
+    std::vector<double> band, lu;
+    std::vector<int> piv;
+    hipStream_t stream = nullptr;
+    double* fwd = nullptr;
+    double* bwd = nullptr;
+    int* dpiv = nullptr;
+    int pivots = 0;
+    double* rhs = nullptr;
+    size_t rhs_cap = 0;
 };
 
 namespace {
@@ -227,6 +257,37 @@ int ensure_seg(ads_ctx* ctx)
 int seg_solve(ads_ctx* ctx, const float* in, float* tmp, float* out, const ads::SegOut& fo = ads::SegOut{})
 {
     ADS_HIP(ctx, ads::launch_seg_solve(ctx->p, in, tmp, out, ctx->seg, fo, ctx->stream));
+    return ADS_OK;
+}
+
+// The device tables of the partitioned solve for every direction, when the
+// lines fit its kernels (at most kPartMaxK partitions of kPartSeg entries).
+int build_part(ads_ctx* ctx)
+{
+    ctx->part_ok = false;
+    for (int d = 0; d < 3; ++d) {
+        ads::PartHost ph;
+        const std::string e = ads::part_tables(ctx->dir[d], ads::kPartSeg, ph);
+        if (!e.empty()) return fail(ctx, ADS_ERR_SINGULAR, "direction " + std::to_string(d) + ": " + e);
+        if (ph.K > ads::kPartMaxK) return ADS_OK;  // the LU sweeps serve these lines
+        double *lr = nullptr, *ur = nullptr, *gs = nullptr, *hs = nullptr;
+        int rc;
+        if ((rc = upload(ctx, &lr, ph.lrow)) || (rc = upload(ctx, &ur, ph.urow)) || (rc = upload(ctx, &gs, ph.gsp)) ||
+            (rc = upload(ctx, &hs, ph.hsp)))
+            return rc;
+        ctx->part[d] = ads::PartDev<double>{lr, ur, gs, hs, ph.n, ph.K, ph.S};
+    }
+    ctx->part_ok = true;
+    return ADS_OK;
+}
+
+// F <- M^-1 F in FP64 (M^-T: the same, M is symmetric; the LU sweeps apply
+// dgbtrs 'T'), by the partitioned solve or the LU sweeps (ads_set_solver).
+int mass_solve64(ads_ctx* ctx, double* F, bool trans)
+{
+    const bool part = ctx->solver == ADS_SOLVER_PARTITIONED || (ctx->solver == ADS_SOLVER_AUTO && ctx->part_ok);
+    if (part) ADS_HIP(ctx, ads::launch_part_solve<double>(ctx->p, F, ctx->part, ctx->stream));
+    else ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, F, ctx->sw64, trans, ctx->stream));
     return ADS_OK;
 }
 
@@ -506,6 +567,48 @@ int read_scalar(ads_ctx* ctx, const double* d, double* h, const char* what)
     return ADS_OK;
 }
 
+// The right-hand side of a device form (Form3DRHS) of the context's
+// coefficients into F; l2 -> l2_d.  Heat: the point values of U_n (kept by
+// ads_form_un(1); the current coefficients when none is kept or nothing has
+// changed since), the gradients of the current coefficients.
+int form_rhs(ads_ctx* ctx, int form, int f0, double dt, double* F, double* l2_d)
+{
+    const bool un_current = !ctx->un_kept || ctx->un_version == ctx->version;
+    int nparts = 0, rc;
+    switch (form) {
+    case ADS_FORM_HEAT:
+        if (!std::isfinite(dt)) return fail(ctx, ADS_ERR_INVALID, "dt is not finite");
+        if (un_current) return assemble<double>(ctx, ads::kFormHeat, 0, ctx->c, F, dt, ctx->kappa, l2_d);
+        if ((rc = zero(ctx, F, ctx->D))) return rc;
+        ADS_HIP(ctx, ads::launch_heat_dual(ctx->p, ctx->un, ctx->c, F, ctx->tb64, dt, ctx->kappa, ctx->l2_part,
+                                           ctx->l2_cap, &nparts, ctx->stream));
+        ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, l2_d, ctx->stream));
+        return ADS_OK;
+    case ADS_FORM_PROJECT:
+        if (f0 < ADS_F0_CONST || f0 > ADS_F0_BUMP3D) return fail(ctx, ADS_ERR_UNSUPPORTED, "unknown f0 kind");
+        return assemble<double>(ctx, ads::kFormProject, f0, nullptr, F, 0.0, nullptr, l2_d);
+    case ADS_FORM_OIL:
+    case ADS_FORM_OIL_PROJECT: {
+        const bool tz = form == ADS_FORM_OIL_PROJECT;
+        if (!ctx->oil.ready) return fail(ctx, ADS_ERR_INVALID, "oil form: call ads_oil_setup first");
+        if (!tz && !std::isfinite(dt)) return fail(ctx, ADS_ERR_INVALID, "dt is not finite");
+        if (!tz && !un_current)
+            return fail(ctx, ADS_ERR_UNSUPPORTED,
+                        "oil form: its device form reads values and gradients of one field, so U_n must be the "
+                        "current coefficients");
+        if ((rc = zero(ctx, F, ctx->D))) return rc;
+        ADS_HIP(ctx, ads::launch_oil(ctx->p, tz, tz ? nullptr : ctx->c, F, ctx->tb64, ctx->oil.dev, ctx->oil.kq,
+                                     tz ? 0.0 : dt, ctx->l2_part, tz ? nullptr : ctx->oil.dr_part, ctx->l2_cap,
+                                     &nparts, ctx->stream));
+        ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, l2_d, ctx->stream));
+        if (!tz) ADS_HIP(ctx, ads::launch_reduce_partials(ctx->oil.dr_part, nparts, ctx->oil.drained, ctx->stream));
+        return ADS_OK;
+    }
+    default:
+        return fail(ctx, ADS_ERR_INVALID, "unknown right-hand-side form");
+    }
+}
+
 }  // namespace
 
 // ---- the allocation-free grid evaluation (ads_eval_grid_noalloc_d) -------
@@ -616,6 +719,7 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
         if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->dir[d].stiff_rows);
     }
     for (int d = 0; d < 3 && !rc; ++d) rc = upload(ctx, &ctx->knots[d], ctx->dir[d].knots);
+    if (!rc) rc = build_part(ctx);
     if (!rc) rc = scratch_address(ctx, &ctx->grid_scratch);
     if (!rc) rc = dev_alloc(ctx, &ctx->c, ctx->D);
     if (!rc) rc = dev_alloc(ctx, &ctx->F, ctx->D);
@@ -706,6 +810,7 @@ int ads_get_matrix(const ads_ctx* ctx, int dim, const double mix[4], double* den
 int ads_set_coeffs(ads_ctx* ctx, const double* host)
 {
     if (check_ctx(ctx) || !host) return ADS_ERR_INVALID;
+    ++ctx->version;
     ADS_HIP(ctx, hipMemcpyAsync(ctx->c, host, sizeof(double) * ctx->D, hipMemcpyHostToDevice, ctx->stream));
     ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ADS_OK;
@@ -754,8 +859,9 @@ int ads_project(ads_ctx* ctx, int f0_kind, double* l2norm)
     if (f0_kind < ADS_F0_CONST || f0_kind > ADS_F0_BUMP3D) return fail(ctx, ADS_ERR_UNSUPPORTED, "unknown f0 kind");
     int rc = assemble<double>(ctx, ads::kFormProject, f0_kind, nullptr, ctx->F, 0.0, nullptr, nullptr);
     if (rc) return rc;
-    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, ctx->sw64, false, ctx->stream));
+    if ((rc = mass_solve64(ctx, ctx->F, false))) return rc;
     std::swap(ctx->c, ctx->F);
+    ++ctx->version;
     return read_l2(ctx, l2norm);
 }
 
@@ -765,6 +871,7 @@ int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm)
     int rc = ads_step_d(ctx, ctx->c, ctx->F, ctx->kappa, dt, prec, refine, ctx->l2);
     if (rc) return rc;
     std::swap(ctx->c, ctx->F);
+    ++ctx->version;
     return read_l2(ctx, l2norm);
 }
 
@@ -794,8 +901,7 @@ int ads_form_rhs_d(ads_ctx* ctx, const double* c_d, const double* kappa_d, doubl
 int ads_solve_d(ads_ctx* ctx, double* F_d, int trans)
 {
     if (check_ctx(ctx) || !F_d) return ADS_ERR_INVALID;
-    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, F_d, ctx->sw64, trans != 0, ctx->stream));
-    return ADS_OK;
+    return mass_solve64(ctx, F_d, trans != 0);
 }
 
 int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double* kappa_d, double dt, int prec,
@@ -811,7 +917,7 @@ int ads_step_d(ads_ctx* ctx, const double* c_in_d, double* c_out_d, const double
                      ? residual<double, double>(ctx, c_in_d, nullptr, nullptr, target, dt, l2_d ? l2_d : ctx->l2)
                      : assemble<double>(ctx, ads::kFormHeat, 0, c_in_d, target, dt, kappa_d, l2_d);
         if (rc) return rc;
-        ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, target, ctx->sw64, false, ctx->stream));
+        if ((rc = mass_solve64(ctx, target, false))) return rc;
         if (target != c_out_d)
             ADS_HIP(ctx, hipMemcpyAsync(c_out_d, target, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
         return ADS_OK;
@@ -839,7 +945,7 @@ int ads_step_adjoint_d(ads_ctx* ctx, const double* c_n_d, const double* lam_next
         // mu = M^-T lam_next
         ADS_HIP(ctx, hipMemcpyAsync(ctx->mu, lam_next_d, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice,
                                     ctx->stream));
-        ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, true, ctx->stream));
+        if ((rc = mass_solve64(ctx, ctx->mu, true))) return rc;
         // lam_out = A(kappa) mu (A symmetric) and dkappa += -dt c_n,e^T K_e mu_e in one pass
         if ((rc = zero(ctx, lam_out_d, ctx->D))) return rc;
         ADS_HIP(ctx, ads::launch_heat_adjoint<double>(ctx->p, ctx->mu, c_n_d, lam_out_d, ctx->tb64, dt, kappa_d,
@@ -1055,6 +1161,7 @@ int ads_multistep(ads_ctx* ctx, const double mix[12], double dt, double* l2norm)
         prev = out[sub];
     }
     std::swap(ctx->c, ctx->F);
+    ++ctx->version;
     return read_l2(ctx, l2norm);
 }
 
@@ -1097,8 +1204,9 @@ int ads_oil_project(ads_ctx* ctx, double* l2norm)
     ADS_HIP(ctx, ads::launch_oil(ctx->p, true, nullptr, ctx->F, ctx->tb64, ctx->oil.dev, ctx->oil.kq, 0.0,
                                  ctx->l2_part, nullptr, ctx->l2_cap, &nparts, ctx->stream));
     ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, ctx->l2, ctx->stream));
-    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, ctx->sw64, false, ctx->stream));
+    if ((rc = mass_solve64(ctx, ctx->F, false))) return rc;
     std::swap(ctx->c, ctx->F);
+    ++ctx->version;
     return read_l2(ctx, l2norm);
 }
 
@@ -1114,8 +1222,9 @@ int ads_oil_step(ads_ctx* ctx, double dt, double* l2norm, double* drained)
                                  ctx->l2_part, ctx->oil.dr_part, ctx->l2_cap, &nparts, ctx->stream));
     ADS_HIP(ctx, ads::launch_reduce_partials(ctx->l2_part, nparts, ctx->l2, ctx->stream));
     ADS_HIP(ctx, ads::launch_reduce_partials(ctx->oil.dr_part, nparts, ctx->oil.drained, ctx->stream));
-    ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, ctx->sw64, false, ctx->stream));
+    if ((rc = mass_solve64(ctx, ctx->F, false))) return rc;
     std::swap(ctx->c, ctx->F);
+    ++ctx->version;
     if ((rc = read_l2(ctx, l2norm))) return rc;
     return read_scalar(ctx, ctx->oil.drained, drained, "drained");
 }
@@ -1153,15 +1262,17 @@ int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double
 int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms_per_launch)
 {
     if (check_ctx(ctx) || !ms_per_launch || launches < 1) return ADS_ERR_INVALID;
-    if (which != ADS_KERNEL_HEAT_FP64 && which != ADS_KERNEL_KRON_RESIDUAL && which != ADS_KERNEL_OIL)
+    if (which < ADS_KERNEL_HEAT_FP64 || which > ADS_KERNEL_SOLVE_SEG_FP32)
         return fail(ctx, ADS_ERR_INVALID, "ads_time_kernel: unknown kernel");
     if (which == ADS_KERNEL_OIL && !ctx->oil.ready)
         return fail(ctx, ADS_ERR_INVALID, "ads_time_kernel: call ads_oil_setup first");
     int rc = ensure_ir_scratch(ctx);
     if (rc) return rc;
+    if (which == ADS_KERNEL_SOLVE_SEG_FP32 && (rc = ensure_seg(ctx))) return rc;
     if (!ctx->mu && (rc = dev_alloc(ctx, &ctx->mu, ctx->D))) return rc;
     const ads::BandSet bs = band_set(ctx);
     if ((rc = zero(ctx, ctx->mu, ctx->D))) return rc;
+    if ((rc = zero(ctx, ctx->f32a, ctx->D))) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ADS_HIP(ctx, hipEventCreate(&e0));
     hipError_t e = hipEventCreate(&e1);
@@ -1178,6 +1289,13 @@ int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms
         else if (which == ADS_KERNEL_OIL)
             e = ads::launch_oil(ctx->p, false, ctx->c, ctx->mu, ctx->tb64, ctx->oil.dev, ctx->oil.kq, dt,
                                 ctx->l2_part, ctx->oil.dr_part, ctx->l2_cap, &nparts, ctx->stream);
+        else if (which == ADS_KERNEL_SOLVE_FP64)
+            e = ctx->part_ok ? ads::launch_part_solve<double>(ctx->p, ctx->mu, ctx->part, ctx->stream)
+                             : ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, false, ctx->stream);
+        else if (which == ADS_KERNEL_SOLVE_FP64_LU)
+            e = ads::launch_kron_solve<double>(ctx->p, ctx->mu, ctx->sw64, false, ctx->stream);
+        else if (which == ADS_KERNEL_SOLVE_SEG_FP32)
+            e = ads::launch_seg_solve(ctx->p, ctx->f32a, ctx->f32b, ctx->f32c, ctx->seg, ads::SegOut{}, ctx->stream);
         else
             e = ads::launch_kron_residual<float, float>(ctx->p, ctx->c, nullptr, nullptr, ctx->f32b, bs, dt,
                                                         ctx->l2_part, ctx->l2_cap, &nparts, ctx->stream);
@@ -1190,6 +1308,190 @@ int ads_time_kernel(ads_ctx* ctx, int which, double dt, int launches, double* ms
     (void)hipEventDestroy(e1);
     if (e != hipSuccess) return hip_fail(ctx, e, "ads_time_kernel");
     *ms_per_launch = static_cast<double>(ms) / launches;
+    return ADS_OK;
+}
+
+int ads_form_un(ads_ctx* ctx, int subun)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    if (subun < 1 || subun > 3) return fail(ctx, ADS_ERR_INVALID, "FormUn: wrong substep (subun must be 1..3)");
+    if (subun > 1) return ADS_OK;  // the gradients always come from the current coefficients
+    int rc;
+    if (!ctx->un && (rc = dev_alloc(ctx, &ctx->un, ctx->D))) return rc;
+    ADS_HIP(ctx, hipMemcpyAsync(ctx->un, ctx->c, sizeof(double) * ctx->D, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->un_kept = true;
+    ctx->un_version = ctx->version;
+    return ADS_OK;
+}
+
+int ads_form_rhs_sub(ads_ctx* ctx, int form, int f0_kind, int direction, int substep, double dt, double* F_host,
+                     double* l2norm)
+{
+    if (check_ctx(ctx) || !F_host) return ADS_ERR_INVALID;
+    if (direction < 1 || direction > 3 || substep < 1 || substep > 3)
+        return fail(ctx, ADS_ERR_INVALID, "Form3DRHS: direction and substep must be 1..3");
+    int rc = form_rhs(ctx, form, f0_kind, dt, ctx->F, ctx->l2);
+    if (rc) return rc;
+    ADS_HIP(ctx, hipMemcpyAsync(F_host, ctx->F, sizeof(double) * ctx->D, hipMemcpyDeviceToHost, ctx->stream));
+    ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return read_l2(ctx, l2norm);
+}
+
+int ads_sub_step(ads_ctx* ctx, int form, int f0_kind, const double mix[4], int direction, int substep, double dt,
+                 double* l2norm)
+{
+    if (check_ctx(ctx) || !mix) return ADS_ERR_INVALID;
+    if (direction < 1 || direction > 3 || substep < 1 || substep > 3)
+        return fail(ctx, ADS_ERR_INVALID, "Sub_Step: direction and substep must be 1..3");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(mix[k])) return fail(ctx, ADS_ERR_INVALID, "Sub_Step: mix must be finite");
+    const bool mass = mix[0] == 1.0 && mix[1] == 0.0 && mix[2] == 0.0 && mix[3] == 0.0;
+    const ads::SweepTables<double>* sw = ctx->sw64;
+    int rc;
+    if (!mass) {
+        // the sets are referenced by pointer: no reallocation within the cap
+        ctx->mixsets.reserve(64);
+        if ((rc = mix_set(ctx, mix, &sw))) return rc;
+    }
+    if ((rc = form_rhs(ctx, form, f0_kind, dt, ctx->F, ctx->l2))) return rc;
+    if (mass) rc = mass_solve64(ctx, ctx->F, false);
+    else ADS_HIP(ctx, ads::launch_kron_solve<double>(ctx->p, ctx->F, sw, false, ctx->stream));
+    if (rc) return rc;
+    std::swap(ctx->c, ctx->F);
+    ++ctx->version;
+    return read_l2(ctx, l2norm);
+}
+
+int ads_last_drained(ads_ctx* ctx, double* drained)
+{
+    if (check_ctx(ctx) || !drained) return ADS_ERR_INVALID;
+    if (!ctx->oil.ready) return fail(ctx, ADS_ERR_INVALID, "ads_last_drained: call ads_oil_setup first");
+    return read_scalar(ctx, ctx->oil.drained, drained, "drained");
+}
+
+int ads_set_solver(ads_ctx* ctx, int solver)
+{
+    if (check_ctx(ctx)) return ADS_ERR_INVALID;
+    if (solver != ADS_SOLVER_AUTO && solver != ADS_SOLVER_LU && solver != ADS_SOLVER_PARTITIONED)
+        return fail(ctx, ADS_ERR_INVALID, "ads_set_solver: unknown solver");
+    if (solver == ADS_SOLVER_PARTITIONED && !ctx->part_ok)
+        return fail(ctx, ADS_ERR_UNSUPPORTED,
+                    "ads_set_solver: the partitioned solve takes lines of at most " +
+                        std::to_string(ads::kPartMaxK * ads::kPartSeg) + " entries");
+    ctx->solver = solver;
+    return ADS_OK;
+}
+
+int ads_host_part_tables(int n, int p, int* K, int* S, double* lrow, double* urow, double* gsp, double* hsp)
+{
+    if (!K || !S) return fail(nullptr, ADS_ERR_INVALID, "ads_host_part_tables: K and S are required");
+    ads::Direction d;
+    std::string e = ads::build_direction(n, p, d);
+    if (!e.empty()) return fail(nullptr, ADS_ERR_INVALID, e);
+    ads::PartHost ph;
+    e = ads::part_tables(d, ads::kPartSeg, ph);
+    if (!e.empty()) return fail(nullptr, ADS_ERR_SINGULAR, e);
+    *K = ph.K;
+    *S = ph.S;
+    if (lrow) std::copy(ph.lrow.begin(), ph.lrow.end(), lrow);
+    if (urow) std::copy(ph.urow.begin(), ph.urow.end(), urow);
+    if (gsp) std::copy(ph.gsp.begin(), ph.gsp.end(), gsp);
+    if (hsp) std::copy(ph.hsp.begin(), ph.hsp.end(), hsp);
+    return ADS_OK;
+}
+
+int ads_matrix_create(const double* U, int n, int p, int nelem, const double mix[4], ads_matrix** out)
+{
+    if (!U || !mix || !out) return fail(nullptr, ADS_ERR_INVALID, "ads_matrix_create: null argument");
+    *out = nullptr;
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(mix[k])) return fail(nullptr, ADS_ERR_INVALID, "ads_matrix_create: mix must be finite");
+    ads_matrix* m = new (std::nothrow) ads_matrix();
+    if (!m) return fail(nullptr, ADS_ERR_NOMEM, "host allocation failed");
+    std::string e = ads::build_direction_knots(U, n, p, nelem, m->dir);
+    if (!e.empty()) {
+        delete m;
+        return fail(nullptr, ADS_ERR_INVALID, "ads_matrix_create: " + e);
+    }
+    for (int k = 0; k < 4; ++k) m->mix[k] = mix[k];
+    ads::mix_band(m->dir, mix, m->band);
+    e = ads::band_lu(m->dir.dofs, p, m->band, m->lu, m->piv);
+    if (!e.empty()) {
+        delete m;
+        return fail(nullptr, ADS_ERR_SINGULAR, "ads_matrix_create: " + e);
+    }
+    *out = m;
+    return ADS_OK;
+}
+
+int ads_matrix_destroy(ads_matrix* m)
+{
+    if (!m) return ADS_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* ptr : {static_cast<void*>(m->fwd), static_cast<void*>(m->bwd), static_cast<void*>(m->dpiv),
+                      static_cast<void*>(m->rhs)})
+        if (ptr) (void)hipFree(ptr);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return ADS_OK;
+}
+
+const char* ads_matrix_last_error(const ads_matrix* m) { return m ? m->err.c_str() : g_init_error.c_str(); }
+
+int ads_matrix_dense(const ads_matrix* m, double* dense)
+{
+    if (!m || !dense) return ADS_ERR_INVALID;
+    ads::dense_matrix(m->dir, m->mix, dense);
+    return ADS_OK;
+}
+
+int ads_matrix_solve(ads_matrix* m, double* rhs, int eqnum)
+{
+    if (!m || (!rhs && eqnum > 0) || eqnum < 0) return ADS_ERR_INVALID;
+    auto mfail = [&](int code, const std::string& msg) {
+        m->err = msg;
+        return code;
+    };
+    if (eqnum == 0) return ADS_OK;
+    const int N = m->dir.dofs, p = m->dir.p;
+    if (p > 4) return mfail(ADS_ERR_UNSUPPORTED, "ads_matrix_solve: the GPU sweeps are built for degrees 1..4");
+    hipError_t e = hipSuccess;
+    if (!m->stream) {
+        std::vector<double> fwd, bwd;
+        std::vector<int> fpiv;
+        ads::sweep_tables(N, p, m->lu, m->piv, fwd, fpiv, bwd);
+        e = hipStreamCreate(&m->stream);
+        if (e == hipSuccess) e = hipMalloc(&m->fwd, sizeof(double) * fwd.size());
+        if (e == hipSuccess) e = hipMalloc(&m->bwd, sizeof(double) * bwd.size());
+        if (e == hipSuccess) e = hipMalloc(&m->dpiv, sizeof(int) * fpiv.size());
+        if (e == hipSuccess) e = hipMemcpy(m->fwd, fwd.data(), sizeof(double) * fwd.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(m->bwd, bwd.data(), sizeof(double) * bwd.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(m->dpiv, fpiv.data(), sizeof(int) * fpiv.size(), hipMemcpyHostToDevice);
+        m->pivots = 0;
+        for (int v : fpiv) m->pivots |= (v != 0);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return mfail(ADS_ERR_HIP, std::string("ads_matrix_solve: set-up: ") + hipGetErrorString(e));
+        }
+    }
+    const size_t count = static_cast<size_t>(N) * eqnum;
+    if (count > m->rhs_cap) {
+        if (m->rhs) (void)hipFree(m->rhs);
+        m->rhs = nullptr;
+        m->rhs_cap = 0;
+        e = hipMalloc(&m->rhs, sizeof(double) * count);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return mfail(ADS_ERR_NOMEM, std::string("ads_matrix_solve: hipMalloc: ") + hipGetErrorString(e));
+        }
+        m->rhs_cap = count;
+    }
+    const ads::SweepTables<double> st{m->fwd, m->dpiv, m->bwd, N, m->pivots};
+    e = hipMemcpyAsync(m->rhs, rhs, sizeof(double) * count, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = ads::launch_rows_solve(p, m->rhs, eqnum, st, false, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rhs, m->rhs, sizeof(double) * count, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) return mfail(ADS_ERR_HIP, std::string("ads_matrix_solve: ") + hipGetErrorString(e));
     return ADS_OK;
 }
 

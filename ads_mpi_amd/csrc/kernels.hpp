@@ -78,6 +78,10 @@ hipError_t launch_reduce_partials(const double* part, int n, double* out, hipStr
 template <typename T>
 hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans, hipStream_t s);
 
+// The 1-D solves of nrows contiguous lines of st.n entries each (line r at
+// F + r*st.n), in place: F <- A^-1 F (A^-T when trans) for the factors in st.
+hipError_t launch_rows_solve(int p, double* F, int nrows, const SweepTables<double>& st, bool trans, hipStream_t s);
+
 // r = b + A c - M x in FP64 with the 1-D banded matrices (constant kappa):
 // A = M - dt*kappa*K  where M = Mz(x)My(x)Mx, K = Mz(x)My(x)Kx + Mz(x)Ky(x)Mx + Kz(x)My(x)Mx.
 // c, x and b may each be nullptr (zero).  x is read in precision TX; r is

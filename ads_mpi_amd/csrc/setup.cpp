@@ -95,8 +95,7 @@ bool gauss_rule(int q, std::vector<double>& x, std::vector<double>& w)
     return true;
 }
 
-// Values and first derivatives of the p+1 B-splines that are non-zero on
-// span i at u (A2.3 with n = 1 derivative).  out[a*2 + k].
+std::string finish_direction(Direction& d, bool open_uniform);
 
 }  // namespace
 
@@ -119,13 +118,45 @@ std::string build_direction(int n, int p, Direction& d)
         const float v = num / denom;
         d.knots[k] = static_cast<double>(v);
     }
+    return finish_direction(d, true);
+}
 
-    // elements: non-empty spans, first DOF = span - p
-    const int m = n + p + 1;
-    for (int i = p; i <= m - p && i + 1 <= m; ++i)
+std::string build_direction_knots(const double* U, int n, int p, int nelem, Direction& d)
+{
+    if (p < 1 || p > kMaxDegree) return "degree must be 1.." + std::to_string(kMaxDegree);
+    if (n < p) return "n must be >= p (at least one element)";
+    if (!U) return "no knot vector";
+    for (int k = 0; k <= n + p + 1; ++k) {
+        if (!std::isfinite(U[k])) return "knot vector has a non-finite entry";
+        if (k > 0 && U[k] < U[k - 1]) return "knot vector is not non-decreasing";
+    }
+    d = Direction();
+    d.n = n;
+    d.p = p;
+    d.q = p + 1;
+    d.dofs = n + 1;
+    d.knots.assign(U, U + n + p + 2);
+    const std::string e = finish_direction(d, false);
+    if (!e.empty()) return e;
+    if (d.nelem != nelem)
+        return "nelem (" + std::to_string(nelem) + ") is not the knot vector's number of non-empty spans (" +
+               std::to_string(d.nelem) + ")";
+    return "";
+}
+
+namespace {
+
+// Everything of a direction after its knots: elements, Gauss rule, basis
+// tables, element matrices, 1-D band matrices and the banded LU of M.
+std::string finish_direction(Direction& d, bool open_uniform)
+{
+    const int n = d.n, p = d.p;
+    // elements: the non-empty spans [U(i), U(i+1)), i = p..n; first DOF = i - p
+    for (int i = p; i <= n; ++i)
         if (d.knots[i] != d.knots[i + 1]) d.first_dof.push_back(i - p);
     d.nelem = static_cast<int>(d.first_dof.size());
-    if (d.nelem != n - p + 1) return "knot vector has repeated interior knots";
+    if (d.nelem < 1) return "knot vector has no non-empty span";
+    if (open_uniform && d.nelem != n - p + 1) return "knot vector has repeated interior knots";
 
     std::vector<double> xg;
     if (!gauss_rule(d.q, xg, d.wts)) return "unsupported quadrature order";
@@ -202,6 +233,8 @@ std::string build_direction(int n, int p, Direction& d)
     // banded LU of the mass matrix
     return band_lu(d.dofs, p, d.mass_band, d.lu, d.piv);
 }
+
+}  // namespace
 
 std::string band_lu(int N, int p, const std::vector<double>& band, std::vector<double>& lu, std::vector<int>& piv)
 {

@@ -48,6 +48,10 @@ struct Direction {
 
 // Builds the direction; returns an empty string or an error message.
 std::string build_direction(int n, int p, Direction& d);
+// The same on a given knot vector U(0:n+p+1) (non-decreasing) with nelem
+// non-empty spans in [U(p), U(n+1)] (BasisData and MKBBT on the caller's
+// knots, src/basis.F90:74-112, src/projection_engine.F90:63-132).
+std::string build_direction_knots(const double* U, int n, int p, int nelem, Direction& d);
 
 // Banded LU with partial pivoting (dgbtf2 semantics): band in the
 // column-major layout of mass_band (half-bandwidth p), result in the dgbtrf

@@ -345,12 +345,15 @@ struct LineChunk {
 // second pass re-reads what the lane itself just wrote, partly from cache).
 // Line L = (i, j) with i in [0, ni) fastest: base = data + i + j*stride_j;
 // line element k at base + k*stride_k.
-template <typename T, int KL, bool TRANS, bool PIV>
+// STAGE: the factor tables are copied to LDS first (lines whose tables fit
+// in kLdsPerGroup); otherwise the recurrences read them from global memory
+// (wave-uniform addresses, served by the caches), so any line length runs.
+template <typename T, int KL, bool TRANS, bool PIV, bool STAGE>
 __global__ __launch_bounds__(256) void k_sweep_lines(T* __restrict__ data, int ni, long stride_j, long stride_k,
                                                      int nlines, SweepTables<T> gst)
 {
     extern __shared__ unsigned char smem[];
-    const SweepTables<T> st = stage_tables<T, KL>(gst, smem);
+    const SweepTables<T> st = STAGE ? stage_tables<T, KL>(gst, smem) : gst;
     const int L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= nlines) return;
     const int i = L % ni, j = L / ni;
@@ -497,15 +500,16 @@ int row_waves(int n)
 }
 
 // two workgroups per CU: about 263 of them cover 256^2 rows in one round
-template <typename T, int KL, bool TRANS, bool PIV>
+// (STAGE as in k_sweep_lines; without it the LDS holds only the tiles)
+template <typename T, int KL, bool TRANS, bool PIV, bool STAGE>
 __global__ __launch_bounds__(kWave* kRowWaves, 2) void k_sweep_rows(T* __restrict__ data, int nrows,
                                                                  SweepTables<T> gst)
 {
     extern __shared__ unsigned char smem[];
-    const SweepTables<T> st = stage_tables<T, KL>(gst, smem);
+    const SweepTables<T> st = STAGE ? stage_tables<T, KL>(gst, smem) : gst;
     const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
     // each wave's tile follows the tables
-    T* tile = reinterpret_cast<T*>(smem + row_table_bytes<T, KL>(gst.n)) +
+    T* tile = reinterpret_cast<T*>(smem + (STAGE ? row_table_bytes<T, KL>(gst.n) : 0)) +
               static_cast<size_t>(wv) * kWave * (RowTile<T>::value + 1);
     const int row0 = (blockIdx.x * nw + wv) * kWave;
     if (row0 >= nrows) return;
@@ -1078,37 +1082,87 @@ __global__ __launch_bounds__(256) void k_add_f2d(const float* __restrict__ a, co
     out[i] = b ? base + static_cast<double>(b[i]) : base;
 }
 
+// LDS layout of one direction's sweep: the tables when they fit (staged),
+// and for rows the wave tiles after them.
+template <typename T, int KL>
+struct SweepPlan {
+    bool stage;
+    int rw;       // waves per row workgroup
+    size_t lds;   // dynamic LDS bytes
+};
+template <typename T, int KL>
+SweepPlan<T, KL> rows_plan(int n)
+{
+    const size_t tile = static_cast<size_t>(kWave) * (RowTile<T>::value + 1) * sizeof(T);
+    const int rw = row_waves<T, KL>(n);
+    if (rw >= 1) return {true, rw, row_table_bytes<T, KL>(n) + static_cast<size_t>(rw) * tile};
+    return {false, kRowWaves, static_cast<size_t>(kRowWaves) * tile};
+}
+template <typename T, int KL>
+SweepPlan<T, KL> lines_plan(int n)
+{
+    const size_t tb = sweep_table_bytes<T, KL>(n);
+    if (tb <= kLdsPerGroup) return {true, 0, tb};
+    return {false, 0, 0};
+}
+
 template <typename T, int KL, bool TRANS, bool PIV>
-void kron_solve_launch(T* F, const SweepTables<T> sw[3], int rw, size_t bx, size_t by, size_t bz, hipStream_t s)
+void rows_launch(T* F, int nrows, const SweepTables<T>& st, hipStream_t s)
+{
+    const SweepPlan<T, KL> pl = rows_plan<T, KL>(st.n);
+    const int per = kWave * pl.rw;
+    const dim3 g((nrows + per - 1) / per);
+    if (pl.stage) hipLaunchKernelGGL((k_sweep_rows<T, KL, TRANS, PIV, true>), g, dim3(per), pl.lds, s, F, nrows, st);
+    else hipLaunchKernelGGL((k_sweep_rows<T, KL, TRANS, PIV, false>), g, dim3(per), pl.lds, s, F, nrows, st);
+}
+
+template <typename T, int KL, bool TRANS, bool PIV>
+void lines_launch(T* F, int ni, long stride_j, long stride_k, int nlines, const SweepTables<T>& st, hipStream_t s)
+{
+    const SweepPlan<T, KL> pl = lines_plan<T, KL>(st.n);
+    const dim3 g((nlines + 255) / 256);
+    if (pl.stage)
+        hipLaunchKernelGGL((k_sweep_lines<T, KL, TRANS, PIV, true>), g, dim3(256), pl.lds, s, F, ni, stride_j,
+                           stride_k, nlines, st);
+    else
+        hipLaunchKernelGGL((k_sweep_lines<T, KL, TRANS, PIV, false>), g, dim3(256), 0, s, F, ni, stride_j, stride_k,
+                           nlines, st);
+}
+
+template <typename T, int KL, bool TRANS, bool PIV>
+void kron_solve_launch(T* F, const SweepTables<T> sw[3], hipStream_t s)
 {
     const int nx = sw[0].n, ny = sw[1].n, nz = sw[2].n;
     const long sxy = static_cast<long>(nx) * ny;
-    const int xrows = ny * nz, ylines = nx * nz, zlines = nx * ny;
-    const int rows_per_block = kWave * rw;
-    dim3 gx((xrows + rows_per_block - 1) / rows_per_block), gy((ylines + 255) / 256), gz((zlines + 255) / 256);
-    hipLaunchKernelGGL((k_sweep_rows<T, KL, TRANS, PIV>), gx, dim3(kWave * rw), bx, s, F, xrows, sw[0]);
-    hipLaunchKernelGGL((k_sweep_lines<T, KL, TRANS, PIV>), gy, dim3(256), by, s, F, nx, sxy, static_cast<long>(nx),
-                       ylines, sw[1]);
-    hipLaunchKernelGGL((k_sweep_lines<T, KL, TRANS, PIV>), gz, dim3(256), bz, s, F, zlines, 0L, sxy, zlines, sw[2]);
+    rows_launch<T, KL, TRANS, PIV>(F, ny * nz, sw[0], s);
+    lines_launch<T, KL, TRANS, PIV>(F, nx, sxy, static_cast<long>(nx), nx * nz, sw[1], s);
+    lines_launch<T, KL, TRANS, PIV>(F, nx * ny, 0L, sxy, nx * ny, sw[2], s);
 }
 
 template <typename T, int KL>
 hipError_t kron_solve_p(T* F, const SweepTables<T> sw[3], bool trans, hipStream_t s)
 {
-    const int nx = sw[0].n, ny = sw[1].n, nz = sw[2].n;
-    const int rw = row_waves<T, KL>(nx);
-    const size_t bx =
-        row_table_bytes<T, KL>(nx) + static_cast<size_t>(rw) * kWave * (RowTile<T>::value + 1) * sizeof(T);
-    const size_t by = sweep_table_bytes<T, KL>(ny), bz = sweep_table_bytes<T, KL>(nz);
-    if (rw < 1 || by > kLdsPerGroup || bz > kLdsPerGroup) return hipErrorInvalidValue;  // line too long for the LDS tables
     // the pivot-free instance when no direction's factorisation has interchanges
     const bool piv = sw[0].pivots || sw[1].pivots || sw[2].pivots;
     if (!trans) {
-        if (!piv) kron_solve_launch<T, KL, false, false>(F, sw, rw, bx, by, bz, s);
-        else kron_solve_launch<T, KL, false, true>(F, sw, rw, bx, by, bz, s);
+        if (!piv) kron_solve_launch<T, KL, false, false>(F, sw, s);
+        else kron_solve_launch<T, KL, false, true>(F, sw, s);
     } else {
-        if (!piv) kron_solve_launch<T, KL, true, false>(F, sw, rw, bx, by, bz, s);
-        else kron_solve_launch<T, KL, true, true>(F, sw, rw, bx, by, bz, s);
+        if (!piv) kron_solve_launch<T, KL, true, false>(F, sw, s);
+        else kron_solve_launch<T, KL, true, true>(F, sw, s);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int KL>
+hipError_t rows_solve_p(T* F, int nrows, const SweepTables<T>& st, bool trans, hipStream_t s)
+{
+    if (!trans) {
+        if (!st.pivots) rows_launch<T, KL, false, false>(F, nrows, st, s);
+        else rows_launch<T, KL, false, true>(F, nrows, st, s);
+    } else {
+        if (!st.pivots) rows_launch<T, KL, true, false>(F, nrows, st, s);
+        else rows_launch<T, KL, true, true>(F, nrows, st, s);
     }
     return hipGetLastError();
 }
@@ -1176,6 +1230,18 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
     case 2: return kron_solve_p<T, 2>(F, sw, trans, s);
     case 3: return kron_solve_p<T, 3>(F, sw, trans, s);
     case 4: return kron_solve_p<T, 4>(F, sw, trans, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_rows_solve(int p, double* F, int nrows, const SweepTables<double>& st, bool trans, hipStream_t s)
+{
+    if (nrows < 1) return hipSuccess;
+    switch (p) {
+    case 1: return rows_solve_p<double, 1>(F, nrows, st, trans, s);
+    case 2: return rows_solve_p<double, 2>(F, nrows, st, trans, s);
+    case 3: return rows_solve_p<double, 3>(F, nrows, st, trans, s);
+    case 4: return rows_solve_p<double, 4>(F, nrows, st, trans, s);
     default: return hipErrorInvalidValue;
     }
 }

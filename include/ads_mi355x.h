@@ -48,6 +48,20 @@ extern "C" {
 #define ADS_KERNEL_HEAT_FP64 0      /* heat RHS assembly, FP64 */
 #define ADS_KERNEL_KRON_RESIDUAL 1  /* fused FP64 Kronecker residual, first pass (x = 0, FP32 out, l2) */
 #define ADS_KERNEL_OIL 2            /* oil right-hand side (after ads_oil_setup) */
+#define ADS_KERNEL_SOLVE_FP64 3     /* the FP64 mass solve in x, y, z (partitioned when the lines fit) */
+#define ADS_KERNEL_SOLVE_FP64_LU 4  /* the same by the FP64 LU sweeps */
+#define ADS_KERNEL_SOLVE_SEG_FP32 5 /* the segmented FP32 solve of ADS_PREC_FP32_IR_SEG */
+
+/* right-hand-side forms (ads_form_rhs_sub, ads_sub_step) */
+#define ADS_FORM_HEAT 0         /* problems/heat/RHS_eq.F90:69-81 (t > 0), diffusivity of ads_set_kappa */
+#define ADS_FORM_PROJECT 1      /* J W v f0(X) with f0 of f0_kind (the t = 0 branch, RHS_eq.F90:82-86) */
+#define ADS_FORM_OIL 2          /* problems/oil/RHS_eq.F90:76-91 at t > 0 (after ads_oil_setup) */
+#define ADS_FORM_OIL_PROJECT 3  /* the oil problem's t = 0 branch, J W v initial_state(X) */
+
+/* 1-D mass solves (ads_set_solver) */
+#define ADS_SOLVER_AUTO 0        /* partitioned where every direction's lines fit it, else LU sweeps */
+#define ADS_SOLVER_LU 1          /* banded LU with partial pivoting (dgbtrf/dgbtrs form), line-sequential */
+#define ADS_SOLVER_PARTITIONED 2 /* no-interchange LU of M, lines cut into partitions solved in parallel */
 
 typedef struct ads_ctx ads_ctx;
 
@@ -147,6 +161,50 @@ int ads_step(ads_ctx* ctx, double dt, int prec, int refine, double* l2norm);
  * column are built once and kept.  l2norm (may be NULL) as ads_step. */
 int ads_multistep(ads_ctx* ctx, const double mix[12], double dt, double* l2norm);
 
+/* FormUn(subun, ads, ads_data) -- src/projection_engine.F90:306-441.  The
+ * point values and gradients are formed inside the assembly kernels, so this
+ * fixes only which coefficients they come from: subun = 1 keeps a device copy
+ * of the current coefficients as U_n, the point values (the reference's Un)
+ * of every later heat right-hand side of ads_form_rhs_sub / ads_sub_step;
+ * subun = 2, 3 (Un13, Un23) keep U_n.  The gradients (dUn) always come from
+ * the current coefficients.  Other subun: ADS_ERR_INVALID (the reference
+ * prints "wrong substep", :336-337). */
+int ads_form_un(ads_ctx* ctx, int subun);
+
+/* Form3DRHS(ads, ads_data, direction, substep, RHS_fun, l2norm) --
+ * src/projection_engine.F90:154-284, with RHS_fun a device form (ADS_FORM_*):
+ * F_host (x fastest, the reference's ads_data%F) receives the right-hand side
+ * of the context's coefficients without advancing them, l2norm (may be NULL)
+ * its l2 sum (sum J W v u^2, with u from U_n; J W v f0^2 for the projection
+ * forms).  ADS_FORM_HEAT: sum J W (v u - dt kappa grad u . grad v), u of U_n
+ * (of the current coefficients when ads_form_un(1) was not called or the state
+ * has not changed since), grad u of the current coefficients.  ADS_FORM_OIL
+ * needs U_n to be the current coefficients (ADS_ERR_UNSUPPORTED otherwise).
+ * direction and substep (1..3) are RHS_fun's arguments; the device forms do
+ * not depend on them. */
+int ads_form_rhs_sub(ads_ctx* ctx, int form, int f0_kind, int direction, int substep, double dt, double* F_host,
+                     double* l2norm);
+
+/* Sub_Step(ads, iter, mix, direction, substep, RHS_fun, ads_data, l2norm,
+ * mierr) -- src/ADS.F90:471-794: the right-hand side of ads_form_rhs_sub,
+ * then the solves with A = mix(1) M + mix(2) K + mix(3) B + mix(4) B^T in x,
+ * y and z (ComputeMatrix + SolveOneDirection per direction; each distinct mix
+ * is factored once and kept, mix = (1, 0, 0, 0) is the mass solve), the result
+ * the new coefficients (F -> R(:,2,2,2), :777-779).  l2norm (may be NULL) as
+ * ads_form_rhs_sub. */
+int ads_sub_step(ads_ctx* ctx, int form, int f0_kind, const double mix[4], int direction, int substep, double dt,
+                 double* l2norm);
+
+/* The drained sum of the last oil right-hand side (ads_oil_step, or
+ * ADS_FORM_OIL in ads_sub_step / ads_form_rhs_sub); synchronises. */
+int ads_last_drained(ads_ctx* ctx, double* drained);
+
+/* The algorithm of the FP64 mass solves (ADS_SOLVER_*; no reference
+ * counterpart: the reference's MUMPS solves one column at a time).  AUTO
+ * (the default) takes the partitioned solve when every direction's lines
+ * have at most 512 entries; PARTITIONED is refused beyond that. */
+int ads_set_solver(ads_ctx* ctx, int solver);
+
 /* Form3DRHS after FormUn (projection_engine.F90:154-284) on the context's
  * coefficients and diffusivity, heat integrand at t > 0, without advancing:
  * F (host, x fastest) = A(kappa) c; l2norm (may be NULL) = sum J W u^2. */
@@ -219,6 +277,16 @@ int ads_oil_get_kq(ads_ctx* ctx, double* kq_host);
  * *w = the warm-up distance. */
 int ads_host_seg_tables(int n, int p, int* w, int* lo, int* hi, float* tab);
 
+/* Host only: the tables of the partitioned FP64 solve for one direction of
+ * SIZE n, degree p (no reference counterpart): the line's n + 1 entries cut
+ * into *K partitions of *S (the last may be shorter); rows j = 0 .. K*S + p
+ * (zero beyond n): lrow[j*p + t] = L(j, j-1-t), urow[j*(p+1)] = 1/U(j, j) and
+ * urow[j*(p+1) + t] = U(j, j+t) of the no-interchange LU of the mass matrix,
+ * gsp[j*p + i] / hsp[j*p + i] the forward / backward recurrence's response at
+ * j to a unit state i at the edge of j's partition.  Arrays may be NULL (to
+ * learn K and S first). */
+int ads_host_part_tables(int n, int p, int* K, int* S, double* lrow, double* urow, double* gsp, double* hsp);
+
 /* curves by InitInputData's random walk with uniforms from SplitMix64(seed)
  * (the reference's own stream is unseeded): cN*cL points each of cx, cy, cz */
 int ads_oil_curves(uint64_t seed, int cN, int cL, double* cx, double* cy, double* cz);
@@ -244,6 +312,30 @@ int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const doub
  * the degree to 4.  All calls on one GPU share that array: calls that
  * overlap on one GPU (other streams, other threads, other contexts) race. */
 int ads_eval_grid_noalloc_d(ads_ctx* ctx, const int res[3], const double lo[3], const double hi[3], double* vals_d);
+
+/* ---- 1-D matrices (ComputeMatrix / SolveOneDirection) -------------------- */
+
+typedef struct ads_matrix ads_matrix;
+
+/* ComputeMatrix(U, p, n, nelem, mix, sprsmtrx) -- src/projection_engine.F90:
+ * 531-544 -> MKBBT (:63-132): the (n+1) x (n+1) matrix mix(1) M + mix(2) K +
+ * mix(3) B + mix(4) B^T of the degree-p B-splines on the knot vector
+ * U(0:n+p+1) (non-decreasing; nelem = its non-empty spans in [U(p), U(n+1)]),
+ * entries by Gauss quadrature as MKBBT sums them, and its banded LU with
+ * partial pivoting (dgbtrf form: the MUMPS analysis and factorisation of
+ * SolveOneDirection, src/ADS.F90:306-361, done once here).  Host work; p =
+ * 1..9.  ADS_ERR_SINGULAR for a zero pivot (the reference stops there). */
+int ads_matrix_create(const double* U, int n, int p, int nelem, const double mix[4], ads_matrix** out);
+int ads_matrix_destroy(ads_matrix* m);
+/* Text of the matrix's last error (of the last failed ads_matrix_create when m is NULL). */
+const char* ads_matrix_last_error(const ads_matrix* m);
+/* the matrix as a dense row-major (n+1) x (n+1) array */
+int ads_matrix_dense(const ads_matrix* m, double* dense);
+/* SolveOneDirection(RHS, eqnum, n, p, sprsmtrx) -- src/ADS.F90:294-379:
+ * rhs (host, column-major (n+1) x eqnum, each column one right-hand side) is
+ * overwritten by the solutions, computed on the GPU (one lane per column, the
+ * LU substitution sweeps; p = 1..4).  Copies in and out and synchronises. */
+int ads_matrix_solve(ads_matrix* m, double* rhs, int eqnum);
 
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */
 

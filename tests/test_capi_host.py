@@ -170,3 +170,92 @@ def test_seg_tables_and_truncated_solve(orc, L, n, p):
         got = seg_emulate(rows, mid, lo, hi, w, p, b)
         assert np.linalg.norm(got - seq) <= 2e-8 * np.linalg.norm(seq)  # warm-up truncation, middle row
         assert np.linalg.norm(got - exact) <= 5e-7 * np.linalg.norm(exact)  # FP32 factors
+
+
+def part_tables(L, n, p):
+    K, S = ctypes.c_int(0), ctypes.c_int(0)
+    _lib.check(L.ads_host_part_tables(n, p, ctypes.byref(K), ctypes.byref(S), None, None, None, None))
+    K, S = K.value, S.value
+    rows = K * S + p + 1
+    lrow, urow = np.zeros(rows * p), np.zeros(rows * (p + 1))
+    gsp, hsp = np.zeros(rows * p), np.zeros(rows * p)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    _lib.check(L.ads_host_part_tables(n, p, ctypes.byref(ctypes.c_int()), ctypes.byref(ctypes.c_int()), dp(lrow),
+                                      dp(urow), dp(gsp), dp(hsp)))
+    return K, S, lrow.reshape(rows, p), urow.reshape(rows, p + 1), gsp.reshape(rows, p), hsp.reshape(rows, p)
+
+
+def part_emulate(K, S, lrow, urow, gsp, hsp, p, b):
+    """float64 restatement of the partitioned kernels (k_part_lines): every
+    partition from zero state, the two edge-state scans, the corrections."""
+    n = b.size
+    v = b.astype(np.float64).copy()
+    E, X = [], []
+    for k in range(K):  # forward, local
+        a, ln = k * S, min(S, n - k * S)
+        h = np.zeros(p)
+        for j in range(a, a + ln):
+            y = v[j] - np.dot(lrow[j], h)
+            h = np.concatenate(([y], h[:-1]))
+            v[j] = y
+        E.append(h.copy())
+    for k in range(K):  # scan from below, correction
+        s = np.zeros(p)
+        for kk in range(k):
+            b_ = (kk + 1) * S
+            s = np.array([E[kk][r] + np.dot(gsp[b_ - 1 - r], s) for r in range(p)])
+        a, ln = k * S, min(S, n - k * S)
+        for j in range(a, a + ln):
+            v[j] += np.dot(gsp[j], s)
+    for k in range(K):  # backward, local
+        a, ln = k * S, min(S, n - k * S)
+        g = np.zeros(p)
+        for j in range(a + ln - 1, a - 1, -1):
+            x = (v[j] - np.dot(urow[j, 1:], g)) * urow[j, 0]
+            g = np.concatenate(([x], g[:-1]))
+            v[j] = x
+        X.append(g.copy())
+    for k in range(K):  # scan from above, correction
+        t = np.zeros(p)
+        for kk in range(K - 1, k, -1):
+            a2 = kk * S
+            t = np.array([X[kk][r] + np.dot(hsp[a2 + r], t) for r in range(p)])
+        a, ln = k * S, min(S, n - k * S)
+        for j in range(a, a + ln):
+            v[j] += np.dot(hsp[j], t)
+    return v
+
+
+@pytest.mark.parametrize("n,p", [(258, 3), (129, 2), (20, 4), (9, 1), (500, 3), (40, 2), (5, 4), (3, 3),
+                                 (386, 3), (97, 2), (64, 4), (511, 1)])
+def test_part_tables_and_partitioned_solve(orc, L, n, p):
+    """The partitioned FP64 solve's tables and algorithm, against the oracle's
+    (independently assembled) mass matrix M and a dense solve:
+      * lrow / urow are LU factors of M without interchanges: L U = M to
+        rounding;
+      * partitions of at most 32 entries, at least 2p (one for short lines);
+      * the partitioned solve (restated in float64 with the tables) equals
+        the dense solve of M to rounding (no truncation) for smooth and
+        random right-hand sides."""
+    N = n + 1
+    K, S, lrow, urow, gsp, hsp = part_tables(L, n, p)
+    assert S <= 32 and (K - 1) * S < N <= K * S
+    assert K == 1 or S >= 2 * p
+    U = orc.knots(n, p)
+    M = orc.band_to_dense(orc.mkbbt_band(U, p, n, n - p + 1, (1, 0, 0, 0)), p, p)
+    Lm, Um = np.eye(N), np.zeros((N, N))
+    for j in range(N):
+        Um[j, j] = 1.0 / urow[j, 0]
+        for s_ in range(1, p + 1):
+            if j - s_ >= 0:
+                Lm[j, j - s_] = lrow[j, s_ - 1]
+            if j + s_ < N:
+                Um[j, j + s_] = urow[j, s_]
+    assert np.abs(Lm @ Um - M).max() <= 1e-14 * np.abs(M).max()
+    assert not lrow[N:].any() and not urow[N:].any() and not gsp[N:].any() and not hsp[N:].any()
+    rng = np.random.default_rng(n * 10 + p)
+    xs = np.linspace(0.0, 1.0, N)
+    for b in (np.sin(3 * xs) + xs ** 2, rng.standard_normal(N)):
+        exact = np.linalg.solve(M, b)
+        got = part_emulate(K, S, lrow, urow, gsp, hsp, p, b)
+        assert np.linalg.norm(got - exact) <= 1e-14 * np.linalg.norm(exact)

@@ -115,17 +115,55 @@ def test_assembly_vs_oracle(orc, n, p):
 
 
 @pytest.mark.parametrize("n,p", [((9, 11, 7), 2), ((6, 7, 8), 3), ((3, 3, 3), 3), ((20, 21, 22), 4),
-                                 ((5, 4, 6), 1), ((70, 5, 66), 2), ((33, 40, 3), 3)])
+                                 ((5, 4, 6), 1), ((70, 5, 66), 2), ((33, 40, 3), 3),
+                                 ((900, 5, 4), 3), ((4, 1100, 5), 4), ((6, 5, 1500), 2), ((2000, 3, 3), 1)])
 def test_solve_vs_oracle(orc, n, p):
-    """Banded substitution sweeps (dgbtrs 'N' and 'T'), including cases whose
-    factorisation pivots ((3,3,3) p=3 and (20..22) p=4)."""
+    """The FP64 mass solve against the oracle's dgbtrf/dgbtrs ('N' and 'T'):
+    the LU substitution sweeps, including cases whose factorisation pivots
+    ((3,3,3) p=3 and (20..22) p=4) and lines whose factor tables do not fit
+    in a workgroup's LDS (901 DOFs at p = 3, 1101 at p = 4, 1501 at p = 2,
+    2001 at p = 1: tables read from global memory), and the partitioned
+    solve (no-interchange LU, lines cut into partitions of at most 32
+    entries) where the lines have at most 512 entries."""
     say("case solve", n, p)
     S = orc.Setup(n, p)
     C = Ctx(n, p)
     try:
         F = np.random.default_rng(8).standard_normal(S.D)
-        assert rel(C.solve(F, 0), S.solve(F)) <= TOL
-        assert rel(C.solve(F, 1), S.solve_T(F)) <= TOL
+        ref, ref_t = S.solve(F), S.solve_T(F)
+        solvers = [_lib.ADS_SOLVER_LU]
+        if max(n) < 512:
+            solvers.append(_lib.ADS_SOLVER_PARTITIONED)
+        else:
+            assert C.L.ads_set_solver(C.h, _lib.ADS_SOLVER_PARTITIONED) == _lib.ADS_ERR_UNSUPPORTED
+        for solver in solvers:
+            _lib.check(C.L.ads_set_solver(C.h, solver), C.h)
+            e, et = rel(C.solve(F, 0), ref), rel(C.solve(F, 1), ref_t)
+            say("solver", solver, "rel err", e, "transposed", et)
+            assert e <= TOL and et <= TOL
+    finally:
+        C.close()
+
+
+@pytest.mark.parametrize("n,p", [((800, 4, 5), 3), ((16, 700, 12), 4), ((4, 6, 1200), 2)])
+def test_long_lines_step_forms(orc, n, p):
+    """A heat step on meshes with one direction of 700..1200 DOFs (factor
+    tables beyond the LDS): the FP64 step against the oracle (1e-12), the
+    FP32 LU-sweep and segmented refinement forms against the FP64 step
+    (1e-10 with two passes)."""
+    say("case long lines", n, p)
+    S = orc.Setup(n, p)
+    C = Ctx(n, p)
+    try:
+        dt = 1e-7
+        c = np.random.default_rng(23).uniform(-1, 1, S.D)
+        ref = C.step_d(c, dt, prec=0)
+        exp, _ = S.step(c, dt)
+        assert rel(ref, exp) <= TOL
+        for prec in (_lib.ADS_PREC_FP32_IR_KRON, _lib.ADS_PREC_FP32_IR_SEG, _lib.ADS_PREC_FP64_KRON):
+            err = rel(C.step_d(c, dt, prec=prec, refine=2), ref)
+            say("prec", prec, "rel err vs FP64 step", err)
+            assert err <= (1e-12 if prec == _lib.ADS_PREC_FP64_KRON else 1e-10)
     finally:
         C.close()
 
@@ -200,6 +238,31 @@ def test_heat_steps_128cubed_p2():
         say("100 steps: worst rel err", worst, "worst l2 rel err", worst_l2)
     finally:
         adss.Cleanup(ads, data)
+
+
+@pytest.mark.parametrize("n,p", [((129, 129, 129), 2), ((258, 66, 40), 3), ((60, 386, 30), 3), ((33, 20, 511), 4)])
+def test_step_both_solvers_vs_oracle(orc, n, p):
+    """One FP64 heat step with a lognormal diffusivity by each FP64 mass solve
+    (LU sweeps, partitioned) against the oracle, and the two against each
+    other; anisotropic line lengths up to 512."""
+    say("case step both solvers", n, p)
+    S = orc.Setup(n, p)
+    C = Ctx(n, p)
+    try:
+        dt = 1e-7
+        c = np.random.default_rng(31).uniform(-1, 1, S.D)
+        kappa = orc.lognormal(9, S.E, 0.0, 0.5)
+        exp, _ = S.step(c, dt, kappa=kappa)
+        got = {}
+        for solver in (_lib.ADS_SOLVER_LU, _lib.ADS_SOLVER_PARTITIONED):
+            _lib.check(C.L.ads_set_solver(C.h, solver), C.h)
+            got[solver] = C.step_d(c, dt, kappa=kappa, prec=0)
+            err = rel(got[solver], exp)
+            say("solver", solver, "rel err", err)
+            assert err <= TOL
+        assert rel(got[_lib.ADS_SOLVER_PARTITIONED], got[_lib.ADS_SOLVER_LU]) <= TOL
+    finally:
+        C.close()
 
 
 def test_bump3d_projection_vs_oracle():
