@@ -1,42 +1,30 @@
 ! ADSS.f90 -- drop-in for module ADSS (src/ADS.F90): initialize, Step,
-! MultiStep, Cleanup and PrintSolution keep their names and arguments; the
-! work runs on the GPU through the C ABI (include/ads_mi355x.h).
+! MultiStep, Sub_Step, SolveOneDirection, Cleanup and PrintSolution keep
+! their names and arguments; the work runs on the GPU through the C ABI
+! (include/ads_mi355x.h).
 !
 ! The per-point callback RHS_fun cannot run on the device.  A problem binds
 ! its callback to a device form once, after initialize:
-!     call RegisterHeatForm(ComputePointForRHS, Dt, ADS_F0_BUMP3D, mierr)
+!     call RegisterHeatForm(ComputePointForRHS, Dt, ADS_F0_BUMP3D, mierr, t=t)
 !     call RegisterProjectionForm(ComputePointForRHS, ADS_F0_CONST, mierr)
 !     call RegisterOilForm(ComputePointForRHS, Dt, ads, cx, cy, cz, cN, cL, &
 !                          pumps, drains, mi, Kqmin, Kqmax, radius, ps, ds, mierr)
-! and Step looks the procedure up by address (c_funloc).  An unregistered
-! procedure makes Step return mierr = ADS_ERR_UNSUPPORTED.
+! and the entry points look the procedure up by address (c_funloc).  An
+! unregistered procedure makes them return mierr = ADS_ERR_UNSUPPORTED.
 module ADSS
    use iso_c_binding
    use Setup, only: ADS_setup, ADS_compute_data
    use Interfaces, only: RHS_fun_int
    use ads_capi
+   use ads_forms
    implicit none
    private
 
-   public :: initialize, Step, MultiStep, Cleanup, PrintSolution
+   public :: initialize, Step, MultiStep, Sub_Step, SolveOneDirection, Cleanup, PrintSolution
    public :: RegisterHeatForm, RegisterProjectionForm, RegisterOilForm, SyncSolution, SetDiffusivity
    public :: ads_sync_every_step, oil_drained
    public :: ADS_F0_CONST, ADS_F0_SINPROD, ADS_F0_BUMP3D, ADS_PREC_FP64, ADS_PREC_FP32_IR, ADS_PREC_FP32_IR_KRON, &
              ADS_PREC_FP64_KRON, ADS_PREC_FP32_IR_SEG
-
-   integer, parameter :: FORM_HEAT = 1, FORM_PROJECTION = 2, FORM_OIL = 3, MAXFORMS = 16
-
-   type form_entry
-      type(c_funptr) :: fptr = c_null_funptr
-      integer :: kind = 0
-      integer(c_int) :: f0 = ADS_F0_CONST
-      real(c_double) :: dt = 0.d0
-      integer(c_int) :: prec = ADS_PREC_FP64
-      integer(c_int) :: refine = 2
-   end type form_entry
-
-   type(form_entry), save :: forms(MAXFORMS)
-   integer, save :: nforms = 0
 
    ! copy the coefficients back into ads_data%F after every Step (the
    ! reference's data contract); set .false. and call SyncSolution to read
@@ -48,13 +36,6 @@ module ADSS
    real(kind=8), save :: oil_drained = 0.d0
 
 contains
-
-   subroutine report(ctx, code, what)
-      type(c_ptr), intent(in) :: ctx
-      integer(c_int), intent(in) :: code
-      character(len=*), intent(in) :: what
-      if (code /= ADS_OK) write(*, '(a,a,i0,a,a)') what, ': ADS error ', code, ': ', ads_error_text(ctx)
-   end subroutine
 
    ! ADSS::initialize (src/ADS.F90:11-107) on one rank
    subroutine initialize(n, p, ads, ads_data, mierr)
@@ -117,45 +98,27 @@ contains
       mierr = 0
    end subroutine initialize
 
-   subroutine add_form(RHS_fun, kind, f0, dt, prec, refine, mierr)
-      procedure(RHS_fun_int) :: RHS_fun
-      integer, intent(in) :: kind
-      integer(c_int), intent(in) :: f0, prec, refine
-      real(c_double), intent(in) :: dt
-      integer(kind=4), intent(out) :: mierr
-      type(c_funptr) :: fp
-      integer :: i
-      fp = c_funloc(RHS_fun)
-      do i = 1, nforms
-         if (c_associated(forms(i)%fptr, fp)) exit
-      end do
-      if (i > nforms) then
-         if (nforms == MAXFORMS) then
-            mierr = ADS_ERR_INVALID
-            return
-         end if
-         nforms = nforms + 1
-         i = nforms
-      end if
-      forms(i) = form_entry(fp, kind, f0, dt, prec, refine)
-      mierr = 0
-   end subroutine
-
    ! Binds a heat integrand (problems/heat/RHS_eq.F90:35-88) to the device
-   ! form: Step(0, ...) projects f0 (t = 0 branch), Step(iter > 0, ...) takes
-   ! one explicit step of length Dt (t > 0 branch).
-   subroutine RegisterHeatForm(RHS_fun, Dt, f0, mierr, prec, refine)
+   ! form: its t == 0 branch projects f0, its t > 0 branch takes one explicit
+   ! step of length Dt.  With t (the problem's own time variable, input_data's
+   ! t) the branch follows t as the integrand's test does; without it Step
+   ! takes the t == 0 branch at iter == 0.
+   subroutine RegisterHeatForm(RHS_fun, Dt, f0, mierr, prec, refine, t)
       procedure(RHS_fun_int) :: RHS_fun
       real(kind=8), intent(in) :: Dt
       integer(c_int), intent(in) :: f0
       integer(kind=4), intent(out) :: mierr
       integer(c_int), intent(in), optional :: prec, refine
+      real(kind=8), intent(in), target, optional :: t
       integer(c_int) :: pr, rf
+      type(c_ptr) :: tp
       pr = ADS_PREC_FP64
       rf = 2
+      tp = c_null_ptr
       if (present(prec)) pr = prec
       if (present(refine)) rf = refine
-      call add_form(RHS_fun, FORM_HEAT, f0, real(Dt, c_double), pr, rf, mierr)
+      if (present(t)) tp = c_loc(t)
+      call add_form(RHS_fun, FORM_HEAT, f0, real(Dt, c_double), pr, rf, tp, mierr)
    end subroutine
 
    ! Binds a projection integrand (problems/L2Projection/RHS_eq.F90:34-71).
@@ -163,7 +126,7 @@ contains
       procedure(RHS_fun_int) :: RHS_fun
       integer(c_int), intent(in) :: f0
       integer(kind=4), intent(out) :: mierr
-      call add_form(RHS_fun, FORM_PROJECTION, f0, 0.d0, ADS_PREC_FP64, 1_c_int, mierr)
+      call add_form(RHS_fun, FORM_PROJECTION, f0, 0.d0, ADS_PREC_FP64, 1_c_int, c_null_ptr, mierr)
    end subroutine
 
    ! Binds the oil integrand (problems/oil/RHS_eq.F90:35-94) to the device
@@ -207,7 +170,7 @@ contains
       end if
       call report(ads%handle, mierr, 'RegisterOilForm')
       if (mierr /= ADS_OK) return
-      call add_form(RHS_fun, FORM_OIL, ADS_F0_CONST, real(Dt, c_double), ADS_PREC_FP64, 1_c_int, mierr)
+      call add_form(RHS_fun, FORM_OIL, ADS_F0_CONST, real(Dt, c_double), ADS_PREC_FP64, 1_c_int, c_null_ptr, mierr)
    end subroutine
 
    ! per-element diffusivity (nelem(1)*nelem(2)*nelem(3), ex fastest)
@@ -227,7 +190,9 @@ contains
       call report(ads%handle, mierr, 'SyncSolution')
    end subroutine
 
-   ! ADSS::Step (src/ADS.F90:437-461)
+   ! ADSS::Step (src/ADS.F90:437-461): FormUn(1) and one Sub_Step with mix
+   ! (1, 0, 0, 0), i.e. the right-hand side solved with the mass matrix in
+   ! x, y and z, as one C-ABI call (the heat form's precision form applies).
    subroutine Step(iter, RHS_fun, ads, ads_data, l2norm, mierr)
       integer(kind=4), intent(in) :: iter
       procedure(RHS_fun_int) :: RHS_fun
@@ -235,35 +200,29 @@ contains
       type(ADS_compute_data), intent(inout) :: ads_data
       real(kind=8), intent(out) :: l2norm
       integer(kind=4), intent(out) :: mierr
-      type(c_funptr) :: fp
-      integer :: i
+      type(form_entry) :: f
       real(c_double) :: l2, dr
       integer(c_int) :: rc
-      fp = c_funloc(RHS_fun)
       l2norm = 0.d0
-      do i = 1, nforms
-         if (c_associated(forms(i)%fptr, fp)) exit
-      end do
-      if (i > nforms) then
-         write(*, '(a)') 'Step: RHS_fun is not bound to a device form (RegisterHeatForm/RegisterOilForm/...)'
+      if (.not. find_form(RHS_fun, 'Step', f)) then
          mierr = ADS_ERR_UNSUPPORTED
          return
       end if
-      if (forms(i)%kind == FORM_OIL) then
+      if (f%kind == FORM_OIL) then
          if (iter == 0) then
             rc = ads_oil_project(ads%handle, l2)
             ads_data%t = 0.d0
          else
-            rc = ads_oil_step(ads%handle, forms(i)%dt, l2, dr)
+            rc = ads_oil_step(ads%handle, f%dt, l2, dr)
             if (rc == ADS_OK) oil_drained = oil_drained + dr
-            ads_data%t = ads_data%t + forms(i)%dt
+            ads_data%t = ads_data%t + f%dt
          end if
-      else if (forms(i)%kind == FORM_PROJECTION .or. iter == 0) then
-         rc = ads_project(ads%handle, forms(i)%f0, l2)
+      else if (f%kind == FORM_PROJECTION .or. form_at_t0(f, iter, ads_data%t)) then
+         rc = ads_project(ads%handle, f%f0, l2)
          ads_data%t = 0.d0
       else
-         rc = ads_step(ads%handle, forms(i)%dt, forms(i)%prec, forms(i)%refine, l2)
-         ads_data%t = ads_data%t + forms(i)%dt
+         rc = ads_step(ads%handle, f%dt, f%prec, f%refine, l2)
+         ads_data%t = ads_data%t + f%dt
       end if
       call report(ads%handle, rc, 'Step')
       mierr = rc
@@ -275,7 +234,8 @@ contains
    ! ADSS::MultiStep (src/ADS.F90:391-429): three substeps, substep s solving
    ! with mix(1,s)M + mix(2,s)K + mix(3,s)B + mix(4,s)B^T in every direction;
    ! the registered heat form supplies the integrand (values of the step's
-   ! initial state, gradients of the latest coefficients).
+   ! initial state, gradients of the latest coefficients).  One C-ABI call:
+   ! the same as FormUn(s) and Sub_Step(..., mix(:, s), s, s, ...) for s = 1..3.
    subroutine MultiStep(iter, mix, RHS_fun, ads, ads_data, l2norm, mierr)
       integer(kind=4), intent(in) :: iter
       real(kind=8), intent(in) :: mix(4, 3)
@@ -284,33 +244,98 @@ contains
       type(ADS_compute_data), intent(inout) :: ads_data
       real(kind=8), intent(out) :: l2norm
       integer(kind=4), intent(out) :: mierr
-      type(c_funptr) :: fp
-      integer :: i
+      type(form_entry) :: f
       real(c_double) :: l2
       integer(c_int) :: rc
-      fp = c_funloc(RHS_fun)
       l2norm = 0.d0
-      do i = 1, nforms
-         if (c_associated(forms(i)%fptr, fp)) exit
-      end do
-      if (i > nforms) then
-         write(*, '(a)') 'MultiStep: RHS_fun is not bound to a device form (RegisterHeatForm)'
+      if (.not. find_form(RHS_fun, 'MultiStep', f)) then
          mierr = ADS_ERR_UNSUPPORTED
          return
       end if
-      if (forms(i)%kind /= FORM_HEAT) then
+      if (f%kind /= FORM_HEAT) then
          write(*, '(a)') 'MultiStep: only the heat form has substeps'
          mierr = ADS_ERR_UNSUPPORTED
          return
       end if
-      rc = ads_multistep(ads%handle, mix, forms(i)%dt, l2)
+      rc = ads_multistep(ads%handle, mix, f%dt, l2)
       call report(ads%handle, rc, 'MultiStep')
       mierr = rc
       if (rc /= ADS_OK) return
-      ads_data%t = ads_data%t + forms(i)%dt
+      ads_data%t = ads_data%t + f%dt
       l2norm = l2
       if (ads_sync_every_step) call SyncSolution(ads, ads_data, mierr)
    end subroutine MultiStep
+
+   ! ADSS::Sub_Step (src/ADS.F90:471-794): Form3DRHS with the device form
+   ! bound to RHS_fun (values of U_n kept by the last FormUn(1), gradients of
+   ! the current coefficients), then the solves with mix(1) M + mix(2) K +
+   ! mix(3) B + mix(4) B^T in x, y and z (ComputeMatrix + SolveOneDirection
+   ! per direction; each distinct mix is factored once and kept), the result
+   ! the new coefficients.  iter = 0 (or the heat form's t == 0 branch)
+   ! projects the initial state instead.
+   subroutine Sub_Step(ads, iter, mix, direction, substep, RHS_fun, ads_data, l2norm, mierr)
+      type(ADS_setup), intent(in) :: ads
+      integer(kind=4), intent(in) :: iter
+      real(kind=8), intent(in) :: mix(4)
+      integer(kind=4), intent(in) :: direction, substep
+      procedure(RHS_fun_int) :: RHS_fun
+      type(ADS_compute_data), intent(inout) :: ads_data
+      real(kind=8), intent(out) :: l2norm
+      integer(kind=4), intent(out) :: mierr
+      type(form_entry) :: f
+      real(c_double) :: l2, dr
+      integer(c_int) :: rc, form
+      l2norm = 0.d0
+      if (.not. find_form(RHS_fun, 'Sub_Step', f)) then
+         mierr = ADS_ERR_UNSUPPORTED
+         return
+      end if
+      select case (f%kind)
+      case (FORM_OIL)
+         form = ADS_FORM_OIL
+         if (iter == 0) form = ADS_FORM_OIL_PROJECT
+      case (FORM_PROJECTION)
+         form = ADS_FORM_PROJECT
+      case default
+         form = ADS_FORM_HEAT
+         if (form_at_t0(f, iter, ads_data%t)) form = ADS_FORM_PROJECT
+      end select
+      rc = ads_sub_step(ads%handle, form, f%f0, mix, int(direction, c_int), int(substep, c_int), f%dt, l2)
+      call report(ads%handle, rc, 'Sub_Step')
+      mierr = rc
+      if (rc /= ADS_OK) return
+      if (form == ADS_FORM_OIL) then
+         rc = ads_last_drained(ads%handle, dr)
+         if (rc == ADS_OK) oil_drained = oil_drained + dr
+      end if
+      l2norm = l2
+      if (ads_sync_every_step) call SyncSolution(ads, ads_data, mierr)
+   end subroutine Sub_Step
+
+   ! ADSS::SolveOneDirection (src/ADS.F90:294-379): RHS(1:n+1, 1:eqnum), each
+   ! column one right-hand side, overwritten by the solution of sprsmtrx's
+   ! system (ComputeMatrix), on the GPU.  The matrix was factored (banded LU
+   ! with partial pivoting) when ComputeMatrix built it; the reference runs
+   ! MUMPS' analysis, factorisation and one solve per column here.
+   subroutine SolveOneDirection(RHS, eqnum, n, p, sprsmtrx)
+      use sparse, only: sparse_matrix, matrix_error_text
+      real(kind=8) :: RHS(:, :)
+      integer(kind=4) :: eqnum
+      integer :: n, p
+      type(sparse_matrix), pointer, intent(in) :: sprsmtrx
+      integer(c_int) :: rc
+      if (.not. associated(sprsmtrx)) then
+         write(*, '(a)') 'SolveOneDirection: no matrix (ComputeMatrix failed)'
+         return
+      end if
+      if (sprsmtrx%n /= n .or. sprsmtrx%p /= p .or. size(RHS, 1) /= n + 1 .or. size(RHS, 2) < eqnum) then
+         write(*, '(a)') 'SolveOneDirection: RHS must be (n+1) x eqnum for the matrix of ComputeMatrix(..., p, n, ...)'
+         return
+      end if
+      if (eqnum < 1) return
+      rc = ads_matrix_solve(sprsmtrx%handle, RHS, int(eqnum, c_int))
+      if (rc /= ADS_OK) write(*, '(a,i0,a,a)') 'SolveOneDirection: ADS error ', rc, ': ', matrix_error_text(sprsmtrx)
+   end subroutine SolveOneDirection
 
    ! ADSS::PrintSolution (src/ADS.F90:955-989): refreshes ads_data%F and
    ! writes step<iter>.vti, the spline sampled on the 31^3 grid of [0,1]^3

@@ -25,13 +25,22 @@ module Setup
       type(c_ptr) :: handle = c_null_ptr
    end type ADS_setup
 
-   ! The reference's Un/Un13/Un23/dUn quadrature-point arrays (and R, F2, F3,
-   ! F_out...) are never formed: FormUn is fused into the assembly kernel and
-   ! the transposes happen on the device.  F holds the coefficients (x fastest,
-   ! F(s1, s2*s3)) after every Step, as in the reference.
+   ! F holds the coefficients (x fastest, F(s1, s2*s3)) after every Step, as
+   ! in the reference, and the right-hand side after Form3DRHS.  The other
+   ! components keep the reference's names and ranks so that problem code
+   ! naming them compiles, but stay unallocated: the transposed copies (F2,
+   ! F3, F*_out), the neighbour buffer R and the quadrature-point arrays (Un,
+   ! Un13, Un23, dUn) are never formed, because FormUn is fused into the
+   ! assembly kernels and the transposes happen on the device.
    type ADS_compute_data
-      real(kind=8), allocatable, dimension(:,:) :: F
+      real(kind=8), allocatable, dimension(:,:) :: F, F2, F3
+      real(kind=8), allocatable, dimension(:,:) :: F_out, F2_out, F3_out
+      real(kind=8), allocatable, dimension(:,:,:,:) :: R
       real(kind=8) :: t = 0.d0
+      real(kind=8), allocatable, dimension(:,:,:,:,:,:) :: Un
+      real(kind=8), allocatable, dimension(:,:,:,:,:,:) :: Un13
+      real(kind=8), allocatable, dimension(:,:,:,:,:,:) :: Un23
+      real(kind=8), allocatable, dimension(:,:,:,:,:,:,:) :: dUn
    end type ADS_compute_data
 
 end module Setup

@@ -8,6 +8,7 @@ module ads_capi
    integer(c_int), parameter :: ADS_F0_CONST = 0, ADS_F0_SINPROD = 1, ADS_F0_BUMP3D = 2
    integer(c_int), parameter :: ADS_PREC_FP64 = 0, ADS_PREC_FP32_IR = 1, ADS_PREC_FP32_IR_KRON = 2, &
                                  ADS_PREC_FP64_KRON = 3, ADS_PREC_FP32_IR_SEG = 4
+   integer(c_int), parameter :: ADS_FORM_HEAT = 0, ADS_FORM_PROJECT = 1, ADS_FORM_OIL = 2, ADS_FORM_OIL_PROJECT = 3
 
    interface
       integer(c_int) function ads_init(n, p, ctx) bind(C, name="ads_init")
@@ -113,6 +114,60 @@ module ads_capi
          type(c_ptr), value :: ctx
          real(c_double), value :: dt
          real(c_double), intent(out) :: l2norm, drained
+      end function
+      integer(c_int) function ads_form_un(ctx, subun) bind(C, name="ads_form_un")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: subun
+      end function
+      integer(c_int) function ads_form_rhs_sub(ctx, form, f0, direction, substep, dt, F, l2norm) &
+            bind(C, name="ads_form_rhs_sub")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: form, f0, direction, substep
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: F(*)
+         real(c_double), intent(out) :: l2norm
+      end function
+      integer(c_int) function ads_sub_step(ctx, form, f0, mix, direction, substep, dt, l2norm) &
+            bind(C, name="ads_sub_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: form, f0, direction, substep
+         real(c_double), intent(in) :: mix(4)
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: l2norm
+      end function
+      integer(c_int) function ads_last_drained(ctx, drained) bind(C, name="ads_last_drained")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: drained
+      end function
+      integer(c_int) function ads_matrix_create(U, n, p, nelem, mix, m) bind(C, name="ads_matrix_create")
+         import :: c_int, c_ptr, c_double
+         real(c_double), intent(in) :: U(*)
+         integer(c_int), value :: n, p, nelem
+         real(c_double), intent(in) :: mix(4)
+         type(c_ptr), intent(out) :: m
+      end function
+      integer(c_int) function ads_matrix_destroy(m) bind(C, name="ads_matrix_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m
+      end function
+      integer(c_int) function ads_matrix_dense(m, dense) bind(C, name="ads_matrix_dense")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: m
+         real(c_double), intent(out) :: dense(*)
+      end function
+      integer(c_int) function ads_matrix_solve(m, rhs, eqnum) bind(C, name="ads_matrix_solve")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: m
+         real(c_double), intent(inout) :: rhs(*)
+         integer(c_int), value :: eqnum
+      end function
+      type(c_ptr) function ads_matrix_last_error(m) bind(C, name="ads_matrix_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: m
       end function
    end interface
 
