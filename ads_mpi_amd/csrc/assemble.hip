@@ -853,7 +853,7 @@ hipError_t oil_p(bool tzero, const double* c, double* F, const Tables<double>& t
 // and the p+1 values).  The sum runs in EvalSpline's order (x outer, z
 // inner; b = bx*by*bz, then val += b*coeff) without fused multiply-adds.
 template <int P>
-__global__ __launch_bounds__(256) void k_eval_grid(const double* __restrict__ c, int sx, int sy,
+__global__ __launch_bounds__(256) void k_eval_grid(const double* __restrict__ c, int sx, int sy, int sz,
                                                    const int* __restrict__ fx, const double* __restrict__ bx, int rx,
                                                    const int* __restrict__ fy, const double* __restrict__ by, int ry,
                                                    const int* __restrict__ fz, const double* __restrict__ bz, int rz,
@@ -868,11 +868,14 @@ __global__ __launch_bounds__(256) void k_eval_grid(const double* __restrict__ c,
     const int iz = static_cast<int>(i / (static_cast<long>(rx) * ry));
     const int x0 = fx[ix], y0 = fy[iy], z0 = fz[iz];
     double val = 0.0;
+    // a direction of lower degree has zero values past its p + 1 entries;
+    // its coefficient index is clamped there (weight 0 times a finite value)
     for (int a = 0; a < P; ++a)
         for (int b = 0; b < P; ++b)
             for (int d = 0; d < P; ++d) {
                 const double w = bx[ix * P + a] * by[iy * P + b] * bz[iz * P + d];
-                const double cv = c[(x0 + a) + static_cast<long>(sx) * ((y0 + b) + static_cast<long>(sy) * (z0 + d))];
+                const int xi = min(x0 + a, sx - 1), yi = min(y0 + b, sy - 1), zi = min(z0 + d, sz - 1);
+                const double cv = c[xi + static_cast<long>(sx) * (yi + static_cast<long>(sy) * zi)];
                 val = val + w * cv;
             }
     out[i] = val;
@@ -898,17 +901,17 @@ template hipError_t launch_assemble<double>(int, int, int, const double*, double
 template hipError_t launch_assemble<float>(int, int, int, const double*, float*, const Tables<float>&, double,
                                            const double*, double*, int, int*, hipStream_t);
 
-hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, const int* fx, const double* bx, int rx,
+hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, int sz, const int* fx, const double* bx, int rx,
                             const int* fy, const double* by, int ry, const int* fz, const double* bz, int rz,
                             double* out, hipStream_t s)
 {
     const long npts = static_cast<long>(rx) * ry * rz;
     const dim3 grid(static_cast<unsigned>((npts + 255) / 256));
     switch (p) {
-    case 1: hipLaunchKernelGGL(k_eval_grid<2>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
-    case 2: hipLaunchKernelGGL(k_eval_grid<3>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
-    case 3: hipLaunchKernelGGL(k_eval_grid<4>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
-    case 4: hipLaunchKernelGGL(k_eval_grid<5>, grid, dim3(256), 0, s, c, sx, sy, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 1: hipLaunchKernelGGL(k_eval_grid<2>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 2: hipLaunchKernelGGL(k_eval_grid<3>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 3: hipLaunchKernelGGL(k_eval_grid<4>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 4: hipLaunchKernelGGL(k_eval_grid<5>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

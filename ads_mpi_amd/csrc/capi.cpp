@@ -48,8 +48,9 @@ struct MixSet {
 
 struct ads_ctx {
     std::string err;
-    int p = 0;
-    ads::Direction dir[3];
+    int p = 0;               // degree of the kernels: the largest of the three
+    ads::Direction dir[3];   // each direction with its own degree (host tables)
+    ads::Direction kdir[3];  // the same widened to degree p (the kernels' tables)
     long D = 0;
     long E = 0;
     hipStream_t own = nullptr;
@@ -238,7 +239,7 @@ int ensure_seg(ads_ctx* ctx)
         std::vector<float> tab;
         int w = 0, lo = 0, hi = 0;
         const std::string e =
-            ads::seg_lu_tables(ctx->dir[d], kSegTol, kSegTolW, ads::kSegPadLo, ads::kSegMax, tab, w, lo, hi);
+            ads::seg_lu_tables(ctx->kdir[d], kSegTol, kSegTolW, ads::kSegPadLo, ads::kSegMax, tab, w, lo, hi);
         if (!e.empty()) return fail(ctx, ADS_ERR_UNSUPPORTED, "segmented form, direction " + std::to_string(d) + ": " + e);
         // device layout: the middle row, kSegPadLo zero rows, rows 0..n-1, kSegPadHi zero rows
         const int n = ctx->dir[d].dofs, nt = 2 * ctx->p + 1;
@@ -265,9 +266,10 @@ int seg_solve(ads_ctx* ctx, const float* in, float* tmp, float* out, const ads::
 int build_part(ads_ctx* ctx)
 {
     ctx->part_ok = false;
+    if (!ads::part_planes_fit(ctx->kdir[0].dofs, ctx->kdir[1].dofs, sizeof(double))) return ADS_OK;  // LU sweeps
     for (int d = 0; d < 3; ++d) {
         ads::PartHost ph;
-        const std::string e = ads::part_tables(ctx->dir[d], ads::kPartSeg, ph);
+        const std::string e = ads::part_tables(ctx->kdir[d], ads::kPartSeg, ph);
         if (!e.empty()) return fail(ctx, ADS_ERR_SINGULAR, "direction " + std::to_string(d) + ": " + e);
         if (ph.K > ads::kPartMaxK) return ADS_OK;  // the LU sweeps serve these lines
         double *lr = nullptr, *ur = nullptr, *gs = nullptr, *hs = nullptr;
@@ -332,6 +334,23 @@ int assemble(ads_ctx* ctx, int form, int f0, const double* c, T* F, double dt, c
 }
 
 int check_ctx(const ads_ctx* ctx) { return ctx ? ADS_OK : ADS_ERR_INVALID; }
+
+// grid_tables of direction d in the kernels' layout: ctx->p + 1 values per
+// point, zero past the direction's own degree
+void grid_tables_k(const ads_ctx* ctx, int d, int res, double lo, double hi, std::vector<int>& first,
+                   std::vector<double>& vals)
+{
+    std::vector<double> own;
+    ads::grid_tables(ctx->dir[d], res, lo, hi, first, own);
+    const int P = ctx->dir[d].p + 1, PK = ctx->p + 1;
+    if (P == PK) {
+        vals.swap(own);
+        return;
+    }
+    vals.assign(static_cast<size_t>(res) * PK, 0.0);
+    for (int i = 0; i < res; ++i)
+        for (int a = 0; a < P; ++a) vals[static_cast<size_t>(i) * PK + a] = own[static_cast<size_t>(i) * P + a];
+}
 
 int read_l2(ads_ctx* ctx, double* l2norm)
 {
@@ -657,24 +676,27 @@ int scratch_address(ads_ctx* ctx, void** out)
 
 struct GridDir {
     const double* U;  // knots on the device
-    int n, res;
+    int n, p, res;    // p: the direction's own degree
     double lo, hi;
-    double* vals;  // res * (p+1)
+    double* vals;  // res * (kernel degree + 1)
     int* first;    // res
 };
 struct GridArgs {
     GridDir d[3];
-    int p;
+    int p;  // the kernels' degree (the largest)
 };
 
 // One thread per (grid point, direction): the point's first basis index
-// and B-spline values, rounded exactly as the host tables (grid_point).
+// and B-spline values, rounded exactly as the host tables (grid_point), zero
+// past the direction's own degree.
 __global__ __launch_bounds__(256) void k_grid_tables(GridArgs g)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const GridDir& a = g.d[blockIdx.y];
     if (i >= a.res) return;
-    ads::grid_point(a.n, g.p, a.U, a.res, a.lo, a.hi, i, a.first + i, a.vals + static_cast<long>(i) * (g.p + 1));
+    double* v = a.vals + static_cast<long>(i) * (g.p + 1);
+    ads::grid_point(a.n, a.p, a.U, a.res, a.lo, a.hi, i, a.first + i, v);
+    for (int k = a.p + 1; k <= g.p; ++k) v[k] = 0.0;
 }
 
 }  // namespace
@@ -685,14 +707,17 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
 {
     if (!n || !p || !out) return fail(nullptr, ADS_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (p[0] != p[1] || p[1] != p[2])
-        return fail(nullptr, ADS_ERR_UNSUPPORTED, "the GPU kernels need the same degree in x, y and z");
-    if (p[0] < 1 || p[0] > 4) return fail(nullptr, ADS_ERR_UNSUPPORTED, "degree must be 1..4 for the GPU kernels");
+    for (int d = 0; d < 3; ++d)
+        if (p[d] < 1 || p[d] > 4)
+            return fail(nullptr, ADS_ERR_UNSUPPORTED, "degree must be 1..4 in each direction for the GPU kernels");
     ads_ctx* ctx = new (std::nothrow) ads_ctx();
     if (!ctx) return fail(nullptr, ADS_ERR_NOMEM, "host allocation failed");
-    ctx->p = p[0];
+    ctx->p = std::max(p[0], std::max(p[1], p[2]));
     for (int d = 0; d < 3; ++d) {
         std::string e = ads::build_direction(n[d], p[d], ctx->dir[d]);
+        // kernels compiled for one degree: a lower-degree direction runs on
+        // tables widened with exact zeros
+        if (e.empty()) e = ads::pad_direction(ctx->dir[d], ctx->p, ctx->kdir[d]);
         if (!e.empty()) {
             int code = e.find("singular") != std::string::npos ? ADS_ERR_SINGULAR : ADS_ERR_INVALID;
             delete ctx;
@@ -713,10 +738,10 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
     }
     ctx->stream = ctx->own;
     for (int d = 0; d < 3 && !rc; ++d) {
-        rc = upload_dir(ctx, ctx->dir[d], ctx->d64[d], ctx->tb64.d[d], ctx->sw64[d]);
-        if (!rc) rc = upload_dir(ctx, ctx->dir[d], ctx->d32[d], ctx->tb32.d[d], ctx->sw32[d]);
-        if (!rc) rc = upload(ctx, &ctx->mass[d], ctx->dir[d].mass_rows);
-        if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->dir[d].stiff_rows);
+        rc = upload_dir(ctx, ctx->kdir[d], ctx->d64[d], ctx->tb64.d[d], ctx->sw64[d]);
+        if (!rc) rc = upload_dir(ctx, ctx->kdir[d], ctx->d32[d], ctx->tb32.d[d], ctx->sw32[d]);
+        if (!rc) rc = upload(ctx, &ctx->mass[d], ctx->kdir[d].mass_rows);
+        if (!rc) rc = upload(ctx, &ctx->stiff[d], ctx->kdir[d].stiff_rows);
     }
     for (int d = 0; d < 3 && !rc; ++d) rc = upload(ctx, &ctx->knots[d], ctx->dir[d].knots);
     if (!rc) rc = build_part(ctx);
@@ -981,7 +1006,7 @@ int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const doub
     if (npts > (1L << 30)) return fail(ctx, ADS_ERR_INVALID, "ads_eval_grid: more than 2^30 points");
     std::vector<int> first[3];
     std::vector<double> tab[3];
-    for (int d = 0; d < 3; ++d) ads::grid_tables(ctx->dir[d], res[d], lo[d], hi[d], first[d], tab[d]);
+    for (int d = 0; d < 3; ++d) grid_tables_k(ctx, d, res[d], lo[d], hi[d], first[d], tab[d]);
     // the tables and the values live only for this call
     void* buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     auto release = [&]() {
@@ -1001,7 +1026,8 @@ int ads_eval_grid(ads_ctx* ctx, const int res[3], const double lo[3], const doub
     }
     if (e == hipSuccess) e = hipMalloc(&buf[6], sizeof(double) * npts);
     if (e == hipSuccess)
-        e = ads::launch_eval_grid(ctx->p, ctx->c, ctx->dir[0].dofs, ctx->dir[1].dofs, static_cast<int*>(buf[0]),
+        e = ads::launch_eval_grid(ctx->p, ctx->c, ctx->dir[0].dofs, ctx->dir[1].dofs, ctx->dir[2].dofs,
+                                  static_cast<int*>(buf[0]),
                                   static_cast<double*>(buf[1]), res[0], static_cast<int*>(buf[2]),
                                   static_cast<double*>(buf[3]), res[1], static_cast<int*>(buf[4]),
                                   static_cast<double*>(buf[5]), res[2], static_cast<double*>(buf[6]), ctx->stream);
@@ -1034,6 +1060,7 @@ int ads_eval_grid_noalloc_d(ads_ctx* ctx, const int res[3], const double lo[3], 
     for (int d = 0; d < 3; ++d) {
         g.d[d].U = ctx->knots[d];
         g.d[d].n = ctx->dir[d].n;
+        g.d[d].p = ctx->dir[d].p;
         g.d[d].res = res[d];
         g.d[d].lo = lo[d];
         g.d[d].hi = hi[d];
@@ -1042,7 +1069,8 @@ int ads_eval_grid_noalloc_d(ads_ctx* ctx, const int res[3], const double lo[3], 
     }
     hipLaunchKernelGGL(k_grid_tables, dim3(static_cast<unsigned>((rmax + 255) / 256), 3), dim3(256), 0, ctx->stream, g);
     ADS_HIP(ctx, hipGetLastError());
-    ADS_HIP(ctx, ads::launch_eval_grid(ctx->p, ctx->c, ctx->dir[0].dofs, ctx->dir[1].dofs, g.d[0].first, g.d[0].vals,
+    ADS_HIP(ctx, ads::launch_eval_grid(ctx->p, ctx->c, ctx->dir[0].dofs, ctx->dir[1].dofs, ctx->dir[2].dofs,
+                                       g.d[0].first, g.d[0].vals,
                                        res[0], g.d[1].first, g.d[1].vals, res[1], g.d[2].first, g.d[2].vals, res[2],
                                        vals_d, ctx->stream));
     return ADS_OK;
@@ -1106,7 +1134,7 @@ int mix_set(ads_ctx* ctx, const double mix[4], const ads::SweepTables<double>** 
     MixSet m{};
     for (int k = 0; k < 4; ++k) m.mix[k] = mix[k];
     for (int d = 0; d < 3; ++d) {
-        const ads::Direction& dir = ctx->dir[d];
+        const ads::Direction& dir = ctx->kdir[d];
         std::vector<double> band, lu, fwd, bwd;
         std::vector<int> piv, fpiv;
         ads::mix_band(dir, mix, band);
@@ -1174,6 +1202,8 @@ int ads_oil_setup(ads_ctx* ctx, const double* cx, const double* cy, const double
     for (int i = 0; i < 6; ++i)
         if (!std::isfinite(params[i])) return fail(ctx, ADS_ERR_INVALID, "ads_oil_setup: parameters must be finite");
     if (params[3] <= 0.0) return fail(ctx, ADS_ERR_INVALID, "ads_oil_setup: radius must be positive");
+    if (ctx->dir[0].p != ctx->dir[1].p || ctx->dir[1].p != ctx->dir[2].p)
+        return fail(ctx, ADS_ERR_UNSUPPORTED, "ads_oil_setup: the oil form needs the same degree in x, y and z");
     auto& o = ctx->oil;
     o.ready = false;
     const long npt = static_cast<long>(cN) * cL;

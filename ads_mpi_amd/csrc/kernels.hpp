@@ -58,10 +58,10 @@ hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, T* la
                                const double* kappa, double* dkappa, double* l2_part, int part_cap, int* nparts,
                                hipStream_t s);
 
-// The spline with coefficients c (x fastest, sx*sy per z plane) at the
-// tensor grid given by per-direction first-index and basis-value tables;
-// out x fastest (rx*ry*rz).
-hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, const int* fx, const double* bx, int rx,
+// The spline with coefficients c (x fastest, sx*sy*sz) at the tensor grid
+// given by per-direction first-index and basis-value tables (p+1 values per
+// point; zero past a lower-degree direction's own); out x fastest (rx*ry*rz).
+hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, int sz, const int* fx, const double* bx, int rx,
                             const int* fy, const double* by, int ry, const int* fz, const double* bz, int rz,
                             double* out, hipStream_t s);
 

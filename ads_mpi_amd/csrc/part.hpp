@@ -17,7 +17,10 @@
 //     s_(k+1)[r] = y_loc_(b_k - 1 - r) + sum_i g_i(b_k - 1 - r) s_k[i]
 //     t_(k-1)[r] = x_loc_(a_k + r)     + sum_i h_i(a_k + r) t_k[i],
 // so the solve is exact (no truncation): it differs from the sequential
-// substitution by rounding only.
+// substitution by rounding only.  The kernels take g and h only at the edge
+// rows (for the scans); the corrections sum_i g_i(j) s_k[i] and
+// sum_i h_i(j) t_k[i] are formed by running the homogeneous recurrences
+// again from the true states, with the factor rows already in registers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,8 +42,8 @@ struct PartHost {
     std::vector<double> lrow, urow, gsp, hsp;
 };
 
-// Partitions of at most smax entries, at least 2p each (one partition for
-// short lines).  Returns an error message for a non-positive pivot.
+// Partitions of smax >= 2p entries, the last one padded past the line's end
+// with zero rows.  Returns an error message for a non-positive pivot.
 std::string part_tables(const Direction& d, int smax, PartHost& t);
 
 // Device copies (one allocation per array, owned by the caller).
@@ -53,10 +56,19 @@ struct PartDev {
     int n, K, S;
 };
 
-// Most entries per partition of the kernels (a lane keeps its partition in
-// registers); lines longer than kPartMaxK * kPartSeg use the LU sweeps.
-constexpr int kPartSeg = 32;
+// Entries per partition (a lane keeps its partition in registers) and most
+// partitions per line; longer lines use the LU sweeps.
+constexpr int kPartSeg = 24;
 constexpr int kPartMaxK = 16;
+
+// The kernels address the 64 lines of a group from its first line with
+// 32-bit byte offsets.  The y lines of a group span at most 64/nx + 1
+// planes: offsets below (nx ny + 64 (nx + ny)) elements, which must stay
+// below 4 GiB.
+inline bool part_planes_fit(long nx, long ny, long elem_bytes)
+{
+    return (nx * ny + 64 * (nx + ny)) * elem_bytes < (1L << 32);
+}
 
 // F <- (Mz (x) My (x) Mx)^-1 F in place, direction by direction (x rows,
 // then y and z lines), with the partitioned solve (M symmetric: also the

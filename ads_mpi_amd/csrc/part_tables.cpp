@@ -14,12 +14,13 @@ std::string part_tables(const Direction& d, int smax, PartHost& t)
     t = PartHost();
     t.n = n;
     t.p = p;
-    // partitions: at most smax entries, at least 2p (one partition when the
-    // line is short)
-    int K = (n + smax - 1) / smax;
-    if (K > 1 && (n + K - 1) / K < 2 * p) K = std::max(1, n / (2 * p));
+    // partitions of exactly smax entries (the last one padded past the line
+    // with zero rows, which the recurrences carry as exact zeros), so the
+    // kernels run a fixed number of steps; smax >= 2p
+    if (smax < 2 * p) return "partitions shorter than twice the degree";
+    const int K = (n + smax - 1) / smax;
     t.K = K;
-    t.S = (n + K - 1) / K;
+    t.S = smax;
     const int rows = K * t.S + p + 1;
 
     // M = L U without interchanges on a dense band copy: a[i][i-j+p] = M(i, j)

@@ -276,6 +276,77 @@ std::string band_lu(int N, int p, const std::vector<double>& band, std::vector<d
     return std::string();
 }
 
+std::string pad_direction(const Direction& d, int pmax, Direction& out)
+{
+    if (pmax < d.p || pmax > kMaxDegree) return "cannot widen degree " + std::to_string(d.p) + " to " +
+                                                 std::to_string(pmax);
+    const int p = d.p, P = p + 1, Q = d.q, PM = pmax + 1, QM = pmax + 1, ne = d.nelem, N = d.dofs;
+    out = Direction();
+    out.n = d.n;
+    out.p = pmax;
+    out.q = QM;
+    out.nelem = ne;
+    out.dofs = N;
+    out.knots = d.knots;
+    out.first_dof = d.first_dof;
+    out.jac = d.jac;
+    // extra quadrature points: zero weight, at the element's midpoint
+    out.wts.assign(QM, 0.0);
+    for (int iq = 0; iq < Q; ++iq) out.wts[iq] = d.wts[iq];
+    out.pts.assign(static_cast<size_t>(ne) * QM, 0.0);
+    out.tab.assign(static_cast<size_t>(ne) * QM * PM * 2, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        const int span = d.first_dof[e] + p;
+        const double mid = 0.5 * (d.knots[span] + d.knots[span + 1]);
+        for (int iq = 0; iq < QM; ++iq) {
+            out.pts[static_cast<size_t>(e) * QM + iq] = iq < Q ? d.pts[static_cast<size_t>(e) * Q + iq] : mid;
+            if (iq >= Q) continue;
+            for (int a = 0; a < P; ++a)
+                for (int k = 0; k < 2; ++k)
+                    out.tab[((static_cast<size_t>(e) * QM + iq) * PM + a) * 2 + k] =
+                        d.tab[((static_cast<size_t>(e) * Q + iq) * P + a) * 2 + k];
+        }
+    }
+    // element matrices: zero rows and columns past p
+    const int nt = PM * (PM + 1) / 2;
+    out.emat.assign(static_cast<size_t>(ne) * PM * 2 * PM, 0.0);
+    out.emat_sym.assign(static_cast<size_t>(ne) * 2 * nt, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < P; ++a)
+            for (int k = 0; k < 2; ++k)
+                for (int b = 0; b < P; ++b) {
+                    const double v = d.emat[((static_cast<size_t>(e) * P + a) * 2 + k) * P + b];
+                    out.emat[((static_cast<size_t>(e) * PM + a) * 2 + k) * PM + b] = v;
+                    if (a <= b) {
+                        const int t = a * PM - a * (a - 1) / 2 + (b - a);
+                        out.emat_sym[(static_cast<size_t>(e) * 2 + k) * nt + t] = v;
+                    }
+                }
+    // band matrices of half-bandwidth pmax (the extra diagonals are zero)
+    const int bw = 2 * p + 1, bwm = 2 * pmax + 1;
+    auto widen = [&](const std::vector<double>& src, std::vector<double>& dst) {
+        dst.assign(static_cast<size_t>(N) * bwm, 0.0);
+        for (int j = 0; j < N; ++j)
+            for (int i = std::max(0, j - p); i <= std::min(N - 1, j + p); ++i)
+                dst[static_cast<size_t>(j) * bwm + (pmax + i - j)] = src[static_cast<size_t>(j) * bw + (p + i - j)];
+    };
+    widen(d.mass_band, out.mass_band);
+    widen(d.stiff_band, out.stiff_band);
+    widen(d.adv_band, out.adv_band);
+    out.mass_rows.assign(static_cast<size_t>(N) * bwm, 0.0);
+    out.stiff_rows.assign(static_cast<size_t>(N) * bwm, 0.0);
+    for (int i = 0; i < N; ++i)
+        for (int t = 0; t < bwm; ++t) {
+            const int j = i - pmax + t;
+            if (j < 0 || j >= N) continue;
+            out.mass_rows[static_cast<size_t>(i) * bwm + t] = out.mass_band[static_cast<size_t>(j) * bwm + (pmax + i - j)];
+            out.stiff_rows[static_cast<size_t>(i) * bwm + t] =
+                out.stiff_band[static_cast<size_t>(j) * bwm + (pmax + i - j)];
+        }
+    // the same factors with kl = ku = pmax: the extra multipliers are zero
+    return band_lu(N, pmax, out.mass_band, out.lu, out.piv);
+}
+
 void mix_band(const Direction& d, const double mix[4], std::vector<double>& band)
 {
     const int p = d.p, bw = 2 * p + 1, N = d.dofs;

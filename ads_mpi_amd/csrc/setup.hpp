@@ -53,6 +53,14 @@ std::string build_direction(int n, int p, Direction& d);
 // knots, src/basis.F90:74-112, src/projection_engine.F90:63-132).
 std::string build_direction_knots(const double* U, int n, int p, int nelem, Direction& d);
 
+// The direction with its tables widened to degree pmax >= d.p, for kernels
+// compiled for one degree in x, y and z: extra quadrature points of zero
+// weight (at the element's midpoint), extra basis functions with zero values
+// and derivatives, element and band matrices with zero rows, columns and
+// diagonals past d.p, and the LU of the widened band (the same factors: its
+// extra multipliers are zero).  Every quantity the kernels form is the same.
+std::string pad_direction(const Direction& d, int pmax, Direction& out);
+
 // Banded LU with partial pivoting (dgbtf2 semantics): band in the
 // column-major layout of mass_band (half-bandwidth p), result in the dgbtrf
 // layout of Direction::lu and 0-based pivot rows.  Returns an error message

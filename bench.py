@@ -8,31 +8,45 @@ quadrature + x/y/z banded mass-matrix sweeps) at 256^3 elements, p = 3
 Prints ONE JSON line.  ``value`` is FP64 DOF-updates/s of the plain step with
 the state resident in HBM (timed with HIP events on the stream the library
 runs on, bracketed by device synchronisation).  Alongside it:
-  * ``fp32_ir``: the precision form as literally specified (FP32 element
-    assembly + FP32 sweeps, then one FP64 Kronecker-residual refinement) on the
-    same state, its speed-up and its error against the FP64 step;
+  * ``fp32_ir``: the precision form as specified (FP32 element assembly + an
+    FP32 segmented solve, then one correction: FP64 Kronecker residual + FP32
+    segmented solve) on the same state, its speed-up and its error against
+    the FP64 step;
   * ``fp32_ir_kron``: the precision form without element assembly (two passes
-    of FP64 Kronecker residual + FP32 sweeps from a zero start; constant kappa);
+    of FP64 Kronecker residual + FP32 LU sweeps from a zero start; constant
+    kappa);
   * ``fp32_ir_seg``: as ``fp32_ir_kron`` with each FP32 solve cut into line
     segments of at most 32 entries solved in parallel, each from zero state
     w entries outside it (the LU recurrences forget their state);
   * ``fp64_kron``: FP64 throughout, the right-hand side from the fused
     Kronecker operator of the 1-D matrices instead of the element assembly
     (exact for a constant kappa, where the Gauss sum over elements and points
-    factorises per direction) + the same FP64 sweeps;
-  * ``grad_fwd``: wall ratio of a checkpointed gradient (recompute + adjoint)
-    to the forward rollout at 96^3 elements, p = 2, 50 steps, checkpoint 10;
+    factorises per direction) + the same FP64 solve;
+  every form carries its speed-up against the FP64 step and against
+  ``fp64_kron`` (the fastest FP64 path for this constant-kappa workload), and
+  whether it stays within 1e-10 of the FP64 step;
+  * ``grad_fwd``: time of the checkpointed gradient (segment recomputation +
+    adjoint steps) over the time of the forward rollout at 96^3 elements,
+    p = 2, 50 steps, checkpoint 10;
+  * ``config5``: the combined large case (384x384x256 elements, p = 3, a
+    lognormal diffusivity field, 40 steps, checkpoint 8): the FP32 forward
+    and adjoint against the FP64 ones (time per step, DOF/s, HBM fraction of
+    its minimum traffic, gradient/forward, errors of the state and both
+    gradients);
   * ``roofline``: the heat assembly kernel (the step's dominant kernel) timed
     alone with HIP events on the library stream (ads_time_kernel), its
-    algorithmic FP64 flops against MI355X's FP64 peak; ``kron_residual`` the
-    same for the fused residual kernel of the fast precision form, by bytes;
+    algorithmic FP64 flops (element-operator form; the point-quadrature count
+    beside it) against MI355X's FP64 peak; ``kron_residual`` the same for the
+    fused residual kernel of the fast precision form, by bytes;
   * ``oil``: the oil problem (problems/oil: kq(X) exp(mi u) permeability,
     pumps, drains) at 128^3 elements, p = 2: step time, its point-quadrature
     kernel against the FP64 peak, and the oracle's faithful step on a 48^3
     sample on the host cores;
   * ``cpu_baseline``: the oracle's reference-structured CPU path (per-point
-    integrand loops, rebuilt 1-D factorisations, per-line solves) on a
-    bounded sample of the same step, extrapolated to DOF/s.
+    integrand loops, rebuilt 1-D factorisations, per-line solves) timed over
+    one whole step of the same workload on the host cores, with the CPU
+    model; the sum-factorised oracle step beside it, and a one-thread
+    sum-factorised step at 128^3, p = 2.
 Data are synthetic: u0 = prod sin(pi x_i) projected, plus U[-0.01, 0.01)
 noise (seed 2) on the coefficients.
 """
@@ -101,36 +115,89 @@ def oil_cpu_baseline(rhs, dt):
             "kind": "port", "sample": "oracle faithful oil step at 48^3 elements, p = 2 (%.2f s)" % t}
 
 
-def cpu_baseline(n, p, dt, budget_s=8.0):
-    """Time the oracle's faithful path on a bounded sample of the step."""
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", else /proc/cpuinfo)."""
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+_ONE_THREAD_STEP = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import oracle as O
+S = O.Setup(129, 2)
+c = np.random.default_rng(1).uniform(-1.0, 1.0, S.D)
+S.step(c, 1e-6)
+t = []
+for _ in range(3):
+    t0 = time.perf_counter(); S.step(c, 1e-6); t.append(time.perf_counter() - t0)
+print(sorted(t)[1])
+"""
+
+
+def cpu_baseline(n, p, dt):
+    """The oracle's faithful path (the reference's loop structure) timed over
+    one whole step of the benchmark workload on all host threads, plus the
+    sum-factorised step on the same threads and, in a child process, the
+    sum-factorised 128^3 p = 2 step on one thread (median of 3)."""
+    import subprocess
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     S = O.Setup(n, p)
-    nez = S.nelem[2]
     c = np.random.default_rng(2).uniform(-1.0, 1.0, S.D)
     t0 = time.perf_counter()
-    S.form_rhs_heat(c, dt, mode=O.MODE_FAITHFUL, ez_range=(0, 1))
-    t1 = time.perf_counter() - t0
-    layers = int(max(1, min(nez, budget_s / max(t1, 1e-3))))
+    S.step(c, dt, mode=O.MODE_FAITHFUL)
+    t_step = time.perf_counter() - t0
     t0 = time.perf_counter()
-    S.form_rhs_heat(c, dt, mode=O.MODE_FAITHFUL, ez_range=(0, layers))
-    t_asm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    S.solve(c, faithful=True)
-    t_solve = time.perf_counter() - t0
-    t_step = t_asm * (nez / layers) + t_solve
+    S.step(c, dt, mode=O.MODE_SUMFACT)
+    t_sf = time.perf_counter() - t0
+    del S
+    one = None
+    try:
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        res = subprocess.run([sys.executable, "-c", _ONE_THREAD_STEP, os.path.join(ROOT, "oracle")], env=env,
+                             capture_output=True, text=True, timeout=120)
+        if res.returncode == 0:
+            t1 = float(res.stdout.strip().splitlines()[-1])
+            one = {"workload": "128^3 elements, p=2, 130^3 DOFs", "mode": "sumfact", "cores": 1,
+                   "s_per_step": t1, "value": 130 ** 3 / t1}
+    except Exception:
+        one = None
     return {
-        "value": S.D / t_step,
+        "value": S_D(n) / t_step,
         "unit": "DOF/s",
         "cores": threads,
         "kind": "port",
-        "sample": ("oracle faithful path (reference loop structure: FormUn fused per element, per-(q,a) "
-                   "integrand, MKBBT+dgbtrf rebuilt per direction, per-line dgbtrs with reorder copies): "
-                   "assembly over %d of %d element layers (%.2f s) extrapolated, plus the full three-direction "
-                   "solve (%.2f s); extrapolated step %.1f s" % (layers, nez, t_asm, t_solve, t_step)),
+        "cpu": cpu_model(),
+        "sample": ("one whole step of the benchmark workload (256^3 elements, p=3) by the oracle's faithful "
+                   "path (reference loop structure: FormUn fused per element, per-(q,a) integrand, "
+                   "MKBBT+dgbtrf rebuilt per direction, per-line dgbtrs with reorder copies): %.1f s" % t_step),
+        "sumfact": {"s_per_step": t_sf, "value": S_D(n) / t_sf, "cores": threads},
+        "one_thread": one,
     }
+
+
+def S_D(n):
+    return (n + 1) ** 3
 
 
 def main():
@@ -207,7 +274,8 @@ def main():
     forms = {}
     for key, prec, refine, what in (
             ("fp32_ir", _lib.ADS_PREC_FP32_IR, 1,
-             "FP32 element assembly + FP32 sweeps, then 1 FP64 Kronecker-residual refinement"),
+             "FP32 element assembly + FP32 segmented solve, then 1 correction: FP64 Kronecker residual + FP32 "
+             "segmented solve"),
             ("fp32_ir_kron", _lib.ADS_PREC_FP32_IR_KRON, 2,
              "zero start, 2 passes of fused FP64 Kronecker residual + FP32 sweeps (no element assembly)"),
             ("fp32_ir_seg", _lib.ADS_PREC_FP32_IR_SEG, 2,
@@ -227,7 +295,15 @@ def main():
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.steps
         forms[key] = {"value": D / (ms * 1e-3), "ms_per_step": ms, "refine": refine, "form": what,
-                      "speedup_vs_fp64": ms64 / ms, "rel_l2_vs_fp64": rel_l2, "max_abs_err_vs_fp64": max_err}
+                      "speedup_vs_fp64": ms64 / ms, "rel_l2_vs_fp64": rel_l2, "max_abs_err_vs_fp64": max_err,
+                      "within_1e-10": bool(rel_l2 <= 1e-10)}
+    # the FP64 Kronecker form is the fastest FP64 path for this constant-kappa
+    # workload: every form's speed against it as well
+    for key in forms:
+        forms[key]["speedup_vs_fp64_kron"] = forms["fp64_kron"]["ms_per_step"] / forms[key]["ms_per_step"]
+    bad = [k for k in forms if not forms[k]["within_1e-10"]]
+    if bad:
+        print("bench.py: forms beyond 1e-10 of the FP64 step: %s" % bad, file=sys.stderr)
 
     # ---- kernels alone (roofline): HIP events around the launches only ----
     ms_k = ctypes.c_double()
@@ -237,6 +313,23 @@ def main():
     _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_KRON_RESIDUAL, dt, 2, ctypes.byref(ms_k)), h)
     _lib.check(L.ads_time_kernel(h, _lib.ADS_KERNEL_KRON_RESIDUAL, dt, args.steps, ctypes.byref(ms_k)), h)
     ms_res = ms_k.value
+    # the mass solves (x, y and z launches together), by bytes: each direction
+    # reads and writes every value once (16 B/DOF in FP64, 8 in FP32)
+    solves = {}
+    for key, which, bpd, what in (
+            ("fp64_partitioned", _lib.ADS_KERNEL_SOLVE_FP64, 48,
+             "k_part_rows + 2 x k_part_lines<double,3> (no-interchange LU, lines in partitions of <= 32 "
+             "solved in parallel, exact edge-state scans)"),
+            ("fp64_lu", _lib.ADS_KERNEL_SOLVE_FP64_LU, 48,
+             "k_sweep_rows + 2 x k_sweep_lines<double,3> (dgbtrf/dgbtrs form, one lane per line)"),
+            ("fp32_segmented", _lib.ADS_KERNEL_SOLVE_SEG_FP32, 24,
+             "k_seg_rows + 2 x k_seg_lines<3,32> (FP32, segments of <= 32 from zero state w = 32 outside)")):
+        _lib.check(L.ads_time_kernel(h, which, dt, 2, ctypes.byref(ms_k)), h)
+        _lib.check(L.ads_time_kernel(h, which, dt, args.steps, ctypes.byref(ms_k)), h)
+        gbs = bpd * D / (ms_k.value * 1e-3) / 1e9
+        solves[key] = {"bound": "hbm", "kernels": what, "ms_per_launch": ms_k.value,
+                       "bytes_per_launch": bpd * D, "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": gbs / PEAK_HBM_GBS}
     if args.profile:
         # calibration dispatch for the PMC byte counters: a copy of 1 GiB whose
         # source was evicted from the 256 MB MALL by 1 GiB written after it
@@ -282,8 +375,64 @@ def main():
         torch.cuda.synchronize()
         t_bwd = time.perf_counter() - t0
         grad = {"workload": "96^3 p=2, 50 steps, checkpoint every 10, loss 0.5||c_T-c_obs||^2",
+                "definition": "time of loss.backward() (segment recomputation + adjoint steps) / time of the "
+                              "forward rollout",
                 "forward_ms": 1e3 * t_fwd, "gradient_ms": 1e3 * t_bwd, "ratio": t_bwd / t_fwd}
         solver.close()
+
+    # ---- the combined large case: 384x384x256, p = 3, kappa field -----------
+    cfg5 = None
+    if not args.no_grad:
+        n5c, p5c, dt5c, steps5c, ck5c = (386, 386, 258), 3, 5e-9, 40, 8
+        solver5 = HeatSolver(n5c, p5c)
+        D5c, E5c = solver5.D, solver5.E
+        c05 = torch.from_numpy(adss.uniform(5, D5c, -1.0, 1.0)).to("cuda")
+        kap5 = torch.from_numpy(adss.lognormal(5, E5c, 0.0, 0.5)).to("cuda")
+        cobs5 = torch.from_numpy(adss.uniform(6, D5c, -1.0, 1.0)).to("cuda")
+        runs = {}
+        for name, kw in (("fp64", dict(prec=_lib.ADS_PREC_FP64)),
+                         ("fp32", dict(prec=_lib.ADS_PREC_FP32_IR_SEG, refine=2,
+                                       adjoint_prec=_lib.ADS_PREC_FP32_IR_SEG, adjoint_refine=2))):
+            # warm-up: a short rollout and its gradient (first-use tables and scratch)
+            cg = c05.clone().requires_grad_(True)
+            kg = kap5.clone().requires_grad_(True)
+            (0.5 * ((heat_rollout(cg, kg, solver5, dt5c, 2, 1, **kw) - cobs5) ** 2).sum()).backward()
+            cg = c05.clone().requires_grad_(True)
+            kg = kap5.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cT = heat_rollout(cg, kg, solver5, dt5c, steps5c, ck5c, **kw)
+            torch.cuda.synchronize()
+            t_f = time.perf_counter() - t0
+            loss = 0.5 * ((cT - cobs5) ** 2).sum()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t_g = time.perf_counter() - t0
+            runs[name] = (cT.detach(), cg.grad, kg.grad, t_f, t_g)
+        relerr = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+        def leg(name, form):
+            cT, gc, gk, t_f, t_g = runs[name]
+            ms = 1e3 * t_f / steps5c
+            bps = 24.0 * D5c / (ms * 1e-3)  # read c and kappa, write c': 24 B per DOF per step
+            return {"form": form, "ms_per_step": ms, "value": D5c / (ms * 1e-3), "unit": "DOF/s",
+                    "hbm_bytes_per_step": 24 * D5c, "hbm_frac": bps / (PEAK_HBM_GBS * 1e9),
+                    "forward_ms": 1e3 * t_f, "gradient_ms": 1e3 * t_g, "grad_fwd": t_g / t_f}
+
+        cfg5 = {"workload": "384x384x256 elements, p=3, 387x387x259 DOFs, lognormal(0, 0.5) kappa (seed 5), "
+                            "dt=5e-9, 40 steps, checkpoint every 8, loss 0.5||c_T-c_obs||^2",
+                "grad_fwd_definition": "time of loss.backward() (segment recomputation + adjoint steps) / time "
+                                       "of the forward rollout",
+                "fp32": leg("fp32", "forward ADS_PREC_FP32_IR_SEG refine 2 (b = A(kappa) c by the FP64 element "
+                                    "assembly, FP32 segmented solves with FP64 mass residuals), adjoint FP32"),
+                "fp64": leg("fp64", "forward and adjoint ADS_PREC_FP64"),
+                "rel_err_vs_fp64": {"state": relerr(runs["fp32"][0], runs["fp64"][0]),
+                                    "grad_c0": relerr(runs["fp32"][1], runs["fp64"][1]),
+                                    "grad_kappa": relerr(runs["fp32"][2], runs["fp64"][2])}}
+        del runs
+        solver5.close()
 
     # ---- oil problem at 128^3, p = 2 ---------------------------------------
     oil = None
@@ -321,7 +470,7 @@ def main():
                            "(seed 1), 2 pumps, 2 drains, mi=10, Kq 1..1000, radius 0.15",
                "value": D5 / (ms_oil * 1e-3), "unit": "DOF/s", "ms_per_step": ms_oil, "l2norm": l2o.value,
                "kq_setup_ms": 1e3 * t_setup,
-               "kernel": {"name": "k_oil<P=3> (FP64)", "bound": "mfma", "ms_per_launch": ms_k.value,
+               "kernel": {"name": "k_oil<P=3> (FP64)", "bound": "fp64_valu", "ms_per_launch": ms_k.value,
                           "flops_per_launch": fl5, "achieved": ach5, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                           "frac": ach5 / PEAK_FP64_TFLOPS},
                "cpu_baseline": None if args.no_cpu_baseline else oil_cpu_baseline(rhs5, dt5)}
@@ -350,12 +499,26 @@ def main():
         "fp32_ir_seg": forms["fp32_ir_seg"],
         "fp64_kron": forms["fp64_kron"],
         "grad_fwd": grad,
+        "config5": cfg5,
         "oil": oil,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "k_heat<double,P=4> (FP64 VALU; MI355X's FP64 vector and matrix peaks are equal)",
+                     "basis": ("element-operator form: per element F_e = (Mz(x)My(x)Mx - dt kappa (Mz(x)My(x)Kx + "
+                               "Mz(x)Ky(x)Mx + Kz(x)My(x)Mx)) c_e with the 1-D element matrices, 8P^4 + 2P^3 FMA = "
+                               "16P^4 + 4P^3 flops per element (equal to the Gauss-point sum: Q = p+1 points "
+                               "integrate the degree-2p integrands exactly)"),
                      "ms_per_launch": ms_asm, "flops_per_launch": flops,
-                     "flops_per_element": 16 * P ** 4 + 4 * P ** 3, "elements_per_launch": E},
+                     "flops_per_element": 16 * P ** 4 + 4 * P ** 3, "elements_per_launch": E,
+                     "quadrature_form": {
+                         "flops_per_element": 36 * P ** 4 + 12 * P ** 3,
+                         "note": ("the sum-factorised point-quadrature count (u and grad u at the Q^3 points, "
+                                  "the integrand, the transposed contraction), which this kernel does not "
+                                  "execute: its equivalent rate exceeds the FP64 peak"),
+                         "equivalent_achieved": E * (36 * P ** 4 + 12 * P ** 3) / (ms_asm * 1e-3) / 1e12,
+                         "equivalent_frac": E * (36 * P ** 4 + 12 * P ** 3) / (ms_asm * 1e-3) / 1e12
+                         / PEAK_FP64_TFLOPS}},
+        "solves": solves,
         "kron_residual": {"bound": "hbm", "achieved": res_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                           "frac": res_gbs / PEAK_HBM_GBS, "kernel": "k_kron_residual<P=4,x=0,r=FP32>",
                           "ms_per_launch": ms_res, "bytes_per_launch": residual_bytes(D)},

@@ -50,7 +50,8 @@ def test_generators_bitwise_equal_oracle(orc):
 def test_init_argument_checks_fail_loudly(L):
     I3 = ctypes.c_int * 3
     h = ctypes.c_void_p()
-    assert L.ads_init(I3(9, 9, 9), I3(2, 3, 2), ctypes.byref(h)) == _lib.ADS_ERR_UNSUPPORTED
+    # degrees 1..4 per direction (unequal ones too); beyond that the kernels are not built
+    assert L.ads_init(I3(9, 9, 9), I3(2, 5, 2), ctypes.byref(h)) == _lib.ADS_ERR_UNSUPPORTED
     assert b"degree" in L.ads_last_error(None)
     assert L.ads_init(I3(9, 9, 9), I3(7, 7, 7), ctypes.byref(h)) == _lib.ADS_ERR_UNSUPPORTED
     assert L.ads_init(I3(1, 9, 9), I3(2, 2, 2), ctypes.byref(h)) == _lib.ADS_ERR_INVALID
@@ -187,7 +188,9 @@ def part_tables(L, n, p):
 
 def part_emulate(K, S, lrow, urow, gsp, hsp, p, b):
     """float64 restatement of the partitioned kernels (k_part_lines): every
-    partition from zero state, the two edge-state scans, the corrections."""
+    partition from zero state; the two edge-state scans, which take the
+    response tables at the partitions' edge rows; the corrections as the
+    homogeneous recurrences started from the true edge states."""
     n = b.size
     v = b.astype(np.float64).copy()
     E, X = [], []
@@ -199,14 +202,17 @@ def part_emulate(K, S, lrow, urow, gsp, hsp, p, b):
             h = np.concatenate(([y], h[:-1]))
             v[j] = y
         E.append(h.copy())
-    for k in range(K):  # scan from below, correction
+    for k in range(K):  # scan from below, homogeneous forward recurrence
         s = np.zeros(p)
         for kk in range(k):
             b_ = (kk + 1) * S
             s = np.array([E[kk][r] + np.dot(gsp[b_ - 1 - r], s) for r in range(p)])
         a, ln = k * S, min(S, n - k * S)
+        d = s.copy()
         for j in range(a, a + ln):
-            v[j] += np.dot(gsp[j], s)
+            e = -np.dot(lrow[j], d)
+            d = np.concatenate(([e], d[:-1]))
+            v[j] += e
     for k in range(K):  # backward, local
         a, ln = k * S, min(S, n - k * S)
         g = np.zeros(p)
@@ -215,14 +221,17 @@ def part_emulate(K, S, lrow, urow, gsp, hsp, p, b):
             g = np.concatenate(([x], g[:-1]))
             v[j] = x
         X.append(g.copy())
-    for k in range(K):  # scan from above, correction
+    for k in range(K):  # scan from above, homogeneous backward recurrence
         t = np.zeros(p)
         for kk in range(K - 1, k, -1):
             a2 = kk * S
             t = np.array([X[kk][r] + np.dot(hsp[a2 + r], t) for r in range(p)])
         a, ln = k * S, min(S, n - k * S)
-        for j in range(a, a + ln):
-            v[j] += np.dot(hsp[j], t)
+        e = t.copy()
+        for j in range(a + ln - 1, a - 1, -1):
+            w = -np.dot(urow[j, 1:], e) * urow[j, 0]
+            e = np.concatenate(([w], e[:-1]))
+            v[j] += w
     return v
 
 
@@ -233,14 +242,14 @@ def test_part_tables_and_partitioned_solve(orc, L, n, p):
     (independently assembled) mass matrix M and a dense solve:
       * lrow / urow are LU factors of M without interchanges: L U = M to
         rounding;
-      * partitions of at most 32 entries, at least 2p (one for short lines);
+      * partitions of 24 entries, the last one padded past the line with
+        zero rows;
       * the partitioned solve (restated in float64 with the tables) equals
         the dense solve of M to rounding (no truncation) for smooth and
         random right-hand sides."""
     N = n + 1
     K, S, lrow, urow, gsp, hsp = part_tables(L, n, p)
-    assert S <= 32 and (K - 1) * S < N <= K * S
-    assert K == 1 or S >= 2 * p
+    assert S == 24 and (K - 1) * S < N <= K * S
     U = orc.knots(n, p)
     M = orc.band_to_dense(orc.mkbbt_band(U, p, n, n - p + 1, (1, 0, 0, 0)), p, p)
     Lm, Um = np.eye(N), np.zeros((N, N))

@@ -148,9 +148,11 @@ def test_fp32_forms_gradient_384x384x256_p3():
             t_fwd, t_bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
             # forward throughput, and the HBM rate of its minimum traffic per
             # step (read c and kappa, write c': 24 B per DOF) against 8 TB/s
+            # gradient/forward: time of loss.backward() (segment recomputation
+            # + adjoint steps) over the time of the forward rollout
             say(name, "forward %.2f ms/step, %.3g DOF/s, %.1f GB/s (%.3f of 8 TB/s); gradient/forward %.2f"
                 % (t_fwd / steps, D * steps / (t_fwd * 1e-3), 24.0 * D * steps / (t_fwd * 1e-3) / 1e9,
-                   24.0 * D * steps / (t_fwd * 1e-3) / 8e12, (t_fwd + t_bwd) / t_fwd))
+                   24.0 * D * steps / (t_fwd * 1e-3) / 8e12, t_bwd / t_fwd))
         for name in ("fp32", "fp32seg"):
             e_state = float(torch.linalg.norm(res[name][0] - res["fp64"][0]) / torch.linalg.norm(res["fp64"][0]))
             e_gc = float(torch.linalg.norm(res[name][1] - res["fp64"][1]) / torch.linalg.norm(res["fp64"][1]))

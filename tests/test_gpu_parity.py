@@ -123,8 +123,8 @@ def test_solve_vs_oracle(orc, n, p):
     ((3,3,3) p=3 and (20..22) p=4) and lines whose factor tables do not fit
     in a workgroup's LDS (901 DOFs at p = 3, 1101 at p = 4, 1501 at p = 2,
     2001 at p = 1: tables read from global memory), and the partitioned
-    solve (no-interchange LU, lines cut into partitions of at most 32
-    entries) where the lines have at most 512 entries."""
+    solve (no-interchange LU, lines cut into partitions of 24 entries)
+    where the lines have at most 16 x 24 = 384 entries."""
     say("case solve", n, p)
     S = orc.Setup(n, p)
     C = Ctx(n, p)
@@ -132,7 +132,7 @@ def test_solve_vs_oracle(orc, n, p):
         F = np.random.default_rng(8).standard_normal(S.D)
         ref, ref_t = S.solve(F), S.solve_T(F)
         solvers = [_lib.ADS_SOLVER_LU]
-        if max(n) < 512:
+        if max(n) < 384:
             solvers.append(_lib.ADS_SOLVER_PARTITIONED)
         else:
             assert C.L.ads_set_solver(C.h, _lib.ADS_SOLVER_PARTITIONED) == _lib.ADS_ERR_UNSUPPORTED
@@ -240,11 +240,12 @@ def test_heat_steps_128cubed_p2():
         adss.Cleanup(ads, data)
 
 
-@pytest.mark.parametrize("n,p", [((129, 129, 129), 2), ((258, 66, 40), 3), ((60, 386, 30), 3), ((33, 20, 511), 4)])
+@pytest.mark.parametrize("n,p", [((129, 129, 129), 2), ((258, 66, 40), 3), ((60, 300, 30), 3), ((33, 20, 383), 4)])
 def test_step_both_solvers_vs_oracle(orc, n, p):
     """One FP64 heat step with a lognormal diffusivity by each FP64 mass solve
     (LU sweeps, partitioned) against the oracle, and the two against each
-    other; anisotropic line lengths up to 512."""
+    other; anisotropic line lengths up to 384 (partitions with and without
+    the double-buffered loads: at most 12 and 16 partitions)."""
     say("case step both solvers", n, p)
     S = orc.Setup(n, p)
     C = Ctx(n, p)
@@ -263,6 +264,66 @@ def test_step_both_solvers_vs_oracle(orc, n, p):
         assert rel(got[_lib.ADS_SOLVER_PARTITIONED], got[_lib.ADS_SOLVER_LU]) <= TOL
     finally:
         C.close()
+
+
+@pytest.mark.parametrize("n,p", [((14, 17, 12), (2, 3, 1)), ((9, 20, 16), (4, 2, 3)), ((30, 8, 10), (1, 1, 4))])
+def test_unequal_degrees_vs_oracle(orc, n, p):
+    """A degree per direction (ADSS::initialize(n(3), p(3), ...),
+    src/ADS.F90:11): the kernels run at the largest degree on tables widened
+    with exact zeros.  Projection, FP64 steps with a diffusivity field (LU
+    and partitioned solves), the FP64 Kronecker form, the FP32 forms,
+    MultiStep, the adjoint and the output grid against the oracle."""
+    import oracle as O
+
+    say("case unequal degrees", n, p)
+    S = O.Setup(n, p)
+    ads, data = adss.initialize(n, p)
+    try:
+        assert list(ads.p) == list(p)
+        l2 = adss.Step(0, adss.HeatRHS(Dt=1e-7, f0=adss.ADS_F0_BUMP3D), ads, data)
+        c0, l2o = S.project(O.F0_BUMP3D, O.MODE_FAITHFUL)
+        assert rel(data.F, c0) <= TOL and abs(l2 - l2o) <= TOL * l2o
+        C = Ctx.__new__(Ctx)
+        C.ads, C.data, C.h, C.L, C.D = ads, data, data.handle, _lib.load(), data.D
+        c = np.random.default_rng(51).uniform(-1, 1, S.D)
+        kappa = orc.lognormal(11, S.E, 0.0, 0.5)
+        dt = 1e-7
+        exp, _ = S.step(c, dt, kappa=kappa, mode=O.MODE_FAITHFUL)
+        for solver in (_lib.ADS_SOLVER_LU, _lib.ADS_SOLVER_PARTITIONED):
+            _lib.check(C.L.ads_set_solver(C.h, solver), C.h)
+            assert rel(C.step_d(c, dt, kappa=kappa, prec=0), exp) <= TOL
+        exp1, _ = S.step(c, dt)
+        ref = C.step_d(c, dt, prec=0)
+        assert rel(ref, exp1) <= TOL
+        assert rel(C.step_d(c, dt, prec=_lib.ADS_PREC_FP64_KRON), exp1) <= TOL
+        for prec in (_lib.ADS_PREC_FP32_IR, _lib.ADS_PREC_FP32_IR_KRON, _lib.ADS_PREC_FP32_IR_SEG):
+            assert rel(C.step_d(c, dt, prec=prec, refine=2), ref) <= 1e-10
+        mixes = [[1.0, 2e-3, 0.0, 0.0], [1.0, 0.0, 0.02, 0.0], [0.5, 1e-3, 0.0, 0.02]]
+        data.F = c
+        adss.MultiStep(1, np.array(mixes).T, adss.HeatRHS(Dt=1e-4, kappa=kappa), ads, data)
+        assert rel(data.F, S.multistep(c, mixes, 1e-4, kappa=kappa)[0]) <= TOL
+        lam = np.random.default_rng(52).standard_normal(S.D)
+        dko = np.zeros(S.E)
+        lo = S.step_adjoint(c, lam, dt, kappa=kappa, dkappa=dko)
+        cd, ld, kd = dev(c), dev(lam), dev(kappa)
+        out, dk = torch.empty_like(cd), torch.zeros(S.E, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(C.L.ads_step_adjoint_d(C.h, vp(cd), vp(ld), vp(kd), dt, 0, 0, vp(out), vp(dk)), C.h)
+        _lib.check(C.L.ads_synchronize(C.h), C.h)
+        assert rel(out.cpu().numpy(), lo) <= TOL and rel(dk.cpu().numpy(), dko) <= TOL
+        # the output grid: the spline at a 9 x 7 x 5 grid, against the oracle's EvalSpline
+        data.F = c
+        params = adss.PlotParams(resx=9, resy=7, resz=5)
+        vals = adss.EvalGrid(data, params)
+        t = [np.linspace(0.0, 1.0, r) for r in (9, 7, 5)]
+        U = [O.knots(n[d], p[d]) for d in range(3)]
+        cf = np.ascontiguousarray(c)
+        expv = np.array([[[O.lib().orc_eval_spline(0, O._d(U[0]), p[0], n[0], O._d(U[1]), p[1], n[1], O._d(U[2]),
+                                                   p[2], n[2], O._d(cf), x, y, z) for x in t[0]] for y in t[1]]
+                         for z in t[2]])
+        assert np.max(np.abs(vals - expv)) <= 1e-14 * np.max(np.abs(expv))
+    finally:
+        adss.Cleanup(ads, data)
 
 
 def test_bump3d_projection_vs_oracle():
