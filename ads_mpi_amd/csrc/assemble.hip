@@ -116,7 +116,21 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 // substep); cn sits in the third ring.
 constexpr int kHeatPlain = 0, kHeatGrad = 1, kHeatDual = 2;
 
-template <typename T, int P, int MODE>
+// A wave's x positions move up by one lane (lane i takes lane i-1's value,
+// lane 0 takes 0): DPP wave_shr:1 on both halves of a double.
+__device__ __forceinline__ double lane_up(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// DPPR (plain mode only): the P x-offsets of an output are summed across
+// lanes (DPP shifts into P^2 accumulators per lane) before the LDS add, so
+// a wave issues P^2 full-width ds_add per element layer instead of P^3 (the
+// lanes whose x position lies past the wave add theirs directly).
+template <typename T, int P, int MODE, bool DPPR = false>
 __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restrict__ c, T* __restrict__ F,
                                                          const T* __restrict__ emx, const T* __restrict__ emy,
                                                          const T* __restrict__ emz, const double* __restrict__ kappa,
@@ -223,6 +237,13 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
 #pragma unroll
             for (int az = 0; az < P; ++az) slot[az] = (ez + az) % P;
             T gacc = T(0);  // cn_e^T K_e c_e (GRAD)
+            A racc[DPPR ? P : 1][DPPR ? P : 1];  // [azo][ayo] at x position lane (DPPR)
+            if constexpr (DPPR) {
+#pragma unroll
+                for (int azo = 0; azo < P; ++azo)
+#pragma unroll
+                    for (int ayo = 0; ayo < P; ++ayo) racc[azo][ayo] = A(0);
+            }
 
 #pragma unroll 1
             for (int axo = 0; axo < P; ++axo) {
@@ -300,13 +321,29 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
                         }
                         l2 = fma(cs[slot[azo]][wv + ayo][lane + axo], om, l2);
                         if constexpr (GRAD) gacc = fma(ns[slot[azo]][wv + ayo][lane + axo], ok, gacc);
-                        // lanes of one wave address distinct columns here; the
-                        // LDS applies one wave's adds in program order
-                        atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], A(fma(s, ok, om)));
+                        const A val = A(fma(s, ok, om));
+                        if constexpr (DPPR) {
+                            if (lane + axo >= kWave) atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], val);
+                            A sh = val;
+#pragma unroll
+                            for (int t = 1; t < P; ++t)
+                                if (axo >= t) sh = lane_up(sh);
+                            racc[azo][ayo] += sh;
+                        } else {
+                            // lanes of one wave address distinct columns here; the
+                            // LDS applies one wave's adds in program order
+                            atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], val);
+                        }
                     }
                     // keep the scheduler from computing every plane's values up front
                     __builtin_amdgcn_sched_barrier(0);
                 }
+            }
+            if constexpr (DPPR) {
+#pragma unroll
+                for (int azo = 0; azo < P; ++azo)
+#pragma unroll
+                    for (int ayo = 0; ayo < P; ++ayo) atomic_add(&fs[slot[azo]][wv + ayo][lane], racc[azo][ayo]);
             }
             if constexpr (GRAD)
                 if (active)
@@ -317,6 +354,287 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat(const double* __restric
         flush_plane(ez);
     }
     // the last KL planes (their contributions finished before the loop's last barrier)
+    for (int z = ez1; z < ez1 + KL; ++z) flush_plane(z);
+
+    const double l2d = wave_sum(double(l2));
+    if (lane == 0) l2w[wv] = l2d;
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRowsWg; ++w) sum += l2w[w];
+        l2_part[blockIdx.x] = sum;
+    }
+}
+
+// LDS ordering between the lanes of one wave
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Position-first heat assembly (the operator of k_heat, its three modes).
+// The y and z contractions do not involve the element's x index, so they are
+// applied once per x position j of a wave's row instead of once per element
+// and x test index:
+//   GM[j](ayo, azo) = sum_{ay,az} My(ayo,ay) Mz(azo,az) c(j, ay, az)
+//   GS[j](ayo, azo) = sum_{ay,az} (Ky(ayo,ay) Mz(azo,az) + My(ayo,ay) Kz(azo,az)) c(j, ay, az)
+// and element e (lane e, DOFs e..e+p along x) takes the x contractions over
+// positions e..e+p:
+//   F_e(axo, q) = sum_b Mx(axo,b) GM[e+b](q)
+//                 - dt kappa_e (sum_b Kx(axo,b) GM[e+b](q) + sum_b Mx(axo,b) GS[e+b](q))
+//   l2_e = sum_{axo,q} c(e+axo, q) sum_b Mx(axo,b) GM[e+b](q)
+// 5 P^3 + 3.5 P^4 multiply-adds per element against 8 P^4 + 2 P^3.  A lane
+// computes GM, GS of its own position; the p positions past the wave's 64
+// (the halo) are computed by the waves together (wave w: azo = w, w+4, ...;
+// lanes = (row, position)) into LDS before a barrier.  Neighbouring
+// positions' values go through a per-wave LDS slot, one (ayo, azo) at a time.
+// kHeatGrad: c is mu, and dkappa[e] += -dt cn_e^T (K_e mu_e) from a second
+// coefficient ring.  kHeatDual: GM from c (values), GM and GS from cn
+// (gradients) for the stiffness part.
+template <typename T, int P, int MODE>
+__global__ __launch_bounds__(kWave* kRowsWg) void k_heat_pf(const double* __restrict__ c, T* __restrict__ F,
+                                                            const T* __restrict__ emx, const T* __restrict__ emy,
+                                                            const T* __restrict__ emz,
+                                                            const double* __restrict__ kappa, T ndt, AsmDims dm,
+                                                            double* __restrict__ l2_part,
+                                                            const double* __restrict__ cn,
+                                                            double* __restrict__ dkappa)
+{
+    constexpr int KL = P - 1, W = kWave + KL, R = kRowsWg + KL, NT = P * (P + 1) / 2;
+    constexpr int NTH = kWave * kRowsWg;
+    constexpr bool GRAD = MODE == kHeatGrad, DUAL = MODE == kHeatDual, TWO = GRAD || DUAL;
+    constexpr int NV = DUAL ? 3 : 2;  // exchanged values per position: GM(c) [, GM(cn)], GS
+    using A = double;
+    __shared__ T cs[P][R][W];                            // coefficient planes, ring by z
+    __shared__ A fs[P][R][W];                            // right-hand-side planes, ring by z
+    __shared__ T ns[TWO ? P : 1][TWO ? R : 1][TWO ? W : 1];  // second field (cn)
+    __shared__ T xb[kRowsWg][kWave][NV];                 // one (ayo, azo) per position
+    __shared__ T hal[kRowsWg][P][KL][P][NV];             // halo: [row][azo][position - 64][ayo][.]
+    __shared__ T eyl[kRowsWg][2 * NT];                   // y element matrices of the rows
+    __shared__ double l2w[kRowsWg];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int xc = tile % dm.xchunks;
+    const int yb = (tile / dm.xchunks) % dm.yblocks;
+    const int seg = tile / (dm.xchunks * dm.yblocks);
+    const int ex0 = xc * kWave, ey0 = yb * kRowsWg;
+    const int ex = ex0 + lane, ey = ey0 + wv;
+    const int ez0 = seg * dm.zseg;
+    const int ez1 = min(ez0 + dm.zseg, dm.nez);
+    const bool row_ok = ey < dm.ney;
+    const bool active = row_ok && ex < dm.nex;
+    const int exc = min(ex, dm.nex - 1), eyc = min(ey, dm.ney - 1);
+    const int sx = dm.sx;
+    const long plane = static_cast<long>(sx) * dm.sy;
+    const int ncol = min(W, sx - ex0);
+    const int nrow = min(R, dm.sy - ey0);
+
+    // the lane's x element matrices (packed upper triangles; zero rows for an
+    // inactive lane, which then adds exact zeros)
+    const T* __restrict__ Ex = emx + static_cast<size_t>(exc) * 2 * NT;
+    const T* __restrict__ Ey = emy + static_cast<size_t>(eyc) * 2 * NT;
+    for (int i = tid; i < kRowsWg * 2 * NT; i += NTH) {
+        const int r = i / (2 * NT);
+        (&eyl[0][0])[i] = emy[static_cast<size_t>(min(ey0 + r, dm.ney - 1)) * 2 * NT + i % (2 * NT)];
+    }
+    // halo role: lane = (row hr, position 64 + hh)
+    const bool hact = lane < KL * kRowsWg;
+    const int hr = hact ? lane / KL : 0, hh = hact ? lane % KL : 0;
+
+    for (int i = tid; i < P * R * W; i += NTH) (&fs[0][0][0])[i] = A(0);
+
+    constexpr int NE = R * W, NPT = (NE + NTH - 1) / NTH;
+    T pc[NPT], pn[TWO ? NPT : 1];
+    auto fetch = [&](int z) {
+        const long off = ex0 + static_cast<long>(sx) * ey0 + plane * z;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * NTH;
+            const int r = i / W, col = i - r * W;
+            const bool in = i < NE && col < ncol && r < nrow;
+            pc[k] = in ? T(c[off + col + static_cast<long>(sx) * r]) : T(0);
+            if constexpr (TWO) pn[k] = in ? T(cn[off + col + static_cast<long>(sx) * r]) : T(0);
+        }
+    };
+    auto stash = [&](int z) {
+        const int slot = z % P;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * NTH;
+            if (i < NE) {
+                const int r = i / W, col = i - r * W;
+                cs[slot][r][col] = pc[k];
+                if constexpr (TWO) ns[slot][r][col] = pn[k];
+            }
+        }
+    };
+    auto flush_plane = [&](int z) {
+        const int slot = z % P;
+        T* dst = F + ex0 + static_cast<long>(sx) * ey0 + plane * z;
+        for (int i = tid; i < R * W; i += NTH) {
+            const int r = i / W, col = i - r * W;
+            if (col < ncol && r < nrow) atomic_add(&dst[col + static_cast<long>(sx) * r], T(fs[slot][r][col]));
+            fs[slot][r][col] = A(0);
+        }
+    };
+    // z then y contraction of a column cc[ay][az] for output plane azo:
+    // gm[ayo] = (My (x) Mz cc)(ayo, azo), gs[ayo] = ((Ky (x) Mz + My (x) Kz) cc)(ayo, azo)
+    auto yz = [&](const T(&cc)[P][P], const T* __restrict__ Ez, const T* my, const T* ky, int azo, T(&gm)[P],
+                  T(&gs)[P]) {
+        T tm[P], tk[P];
+#pragma unroll
+        for (int ay = 0; ay < P; ++ay) {
+            T m = Ez[tri<P>(azo, 0)] * cc[ay][0];
+            T k = Ez[NT + tri<P>(azo, 0)] * cc[ay][0];
+#pragma unroll
+            for (int az = 1; az < P; ++az) {
+                m = fma(Ez[tri<P>(azo, az)], cc[ay][az], m);
+                k = fma(Ez[NT + tri<P>(azo, az)], cc[ay][az], k);
+            }
+            tm[ay] = m;
+            tk[ay] = k;
+        }
+#pragma unroll
+        for (int ayo = 0; ayo < P; ++ayo) {
+            T m = my[tri<P>(ayo, 0)] * tm[0];
+            T k = fma(ky[tri<P>(ayo, 0)], tm[0], my[tri<P>(ayo, 0)] * tk[0]);
+#pragma unroll
+            for (int ay = 1; ay < P; ++ay) {
+                m = fma(my[tri<P>(ayo, ay)], tm[ay], m);
+                k = fma(ky[tri<P>(ayo, ay)], tm[ay], k);
+                k = fma(my[tri<P>(ayo, ay)], tk[ay], k);
+            }
+            gm[ayo] = m;
+            gs[ayo] = k;
+        }
+    };
+    // the exchanged values of a column at (row r0, position col) for plane azo:
+    // v[ayo][0] = GM(c), [1] = GS (of c, or of cn in DUAL), [2] = GM(cn) (DUAL)
+    auto column_values = [&](int r0, int col, const T* __restrict__ Ez, const T* my, const T* ky, int azo,
+                             const int (&slot)[P], T(&v)[P][NV]) {
+        T cc[P][P];
+#pragma unroll
+        for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+            for (int az = 0; az < P; ++az) cc[ay][az] = cs[slot[az]][r0 + ay][col];
+        T gm[P], gs[P];
+        yz(cc, Ez, my, ky, azo, gm, gs);
+        if constexpr (DUAL) {
+#pragma unroll
+            for (int ay = 0; ay < P; ++ay)
+#pragma unroll
+                for (int az = 0; az < P; ++az) cc[ay][az] = ns[slot[az]][r0 + ay][col];
+            T gmn[P], gsn[P];
+            yz(cc, Ez, my, ky, azo, gmn, gsn);
+#pragma unroll
+            for (int ayo = 0; ayo < P; ++ayo) {
+                v[ayo][0] = gm[ayo];
+                v[ayo][1] = gsn[ayo];
+                v[ayo][2] = gmn[ayo];
+            }
+        } else {
+#pragma unroll
+            for (int ayo = 0; ayo < P; ++ayo) {
+                v[ayo][0] = gm[ayo];
+                v[ayo][1] = gs[ayo];
+            }
+        }
+    };
+
+    for (int z = ez0; z < ez0 + KL; ++z) {
+        fetch(z);
+        stash(z);
+    }
+    fetch(ez0 + KL);
+
+    T l2 = T(0);
+    for (int ez = ez0; ez < ez1; ++ez) {
+        stash(ez + KL);
+        if (ez + 1 < ez1) fetch(ez + KL + 1);
+        __syncthreads();
+        const T* __restrict__ Ez = emz + static_cast<size_t>(ez) * 2 * NT;
+        int slot[P];
+#pragma unroll
+        for (int az = 0; az < P; ++az) slot[az] = (ez + az) % P;
+        // the halo positions' values (rows of every wave; this wave's planes)
+        for (int azo = wv; azo < P; azo += kRowsWg) {
+            T v[P][NV];
+            column_values(hr, kWave + hh, Ez, &eyl[hr][0], &eyl[hr][NT], azo, slot, v);
+            if (hact) {
+#pragma unroll
+                for (int ayo = 0; ayo < P; ++ayo)
+#pragma unroll
+                    for (int k = 0; k < NV; ++k) hal[hr][azo][hh][ayo][k] = v[ayo][k];
+            }
+        }
+        __syncthreads();
+        if (row_ok) {
+            const size_t eidx = exc + static_cast<size_t>(dm.nex) * (eyc + static_cast<size_t>(dm.ney) * ez);
+            const T kap = kappa ? T(kappa[eidx]) : T(1);
+            const T s = ndt * kap;
+            T gacc = T(0);  // cn_e^T K_e mu_e (GRAD)
+            // read per layer (cache hits): kept across the layers they would
+            // take 2 P(P+1)/2 VGPRs through the halo pass too
+            const T* ex_ = Ex;
+            asm volatile("" : "+v"(ex_));
+            T mxs[NT], kxs[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                mxs[t] = active ? ex_[t] : T(0);
+                kxs[t] = active ? ex_[NT + t] : T(0);
+            }
+#pragma unroll 1
+            for (int azo = 0; azo < P; ++azo) {
+                T vo[P][NV];
+                column_values(wv, lane, Ez, Ey, Ey + NT, azo, slot, vo);
+#pragma unroll
+                for (int ayo = 0; ayo < P; ++ayo) {
+#pragma unroll
+                    for (int k = 0; k < NV; ++k) xb[wv][lane][k] = vo[ayo][k];
+                    wave_lds_sync();
+                    T g[P][NV];
+#pragma unroll
+                    for (int k = 0; k < NV; ++k) g[0][k] = vo[ayo][k];
+#pragma unroll
+                    for (int b = 1; b < P; ++b) {
+                        const int src = lane + b;
+                        const T* q = src < kWave ? &xb[wv][src][0] : &hal[wv][azo][src - kWave][ayo][0];
+#pragma unroll
+                        for (int k = 0; k < NV; ++k) g[b][k] = q[k];
+                    }
+                    wave_lds_sync();
+#pragma unroll
+                    for (int axo = 0; axo < P; ++axo) {
+                        // om = (Mx GM(c)), ok = (Kx GM(cn|c) + Mx GS)
+                        constexpr int KG = DUAL ? 2 : 0;
+                        T om = mxs[tri<P>(axo, 0)] * g[0][0];
+                        T ok = fma(kxs[tri<P>(axo, 0)], g[0][KG], mxs[tri<P>(axo, 0)] * g[0][1]);
+#pragma unroll
+                        for (int b = 1; b < P; ++b) {
+                            om = fma(mxs[tri<P>(axo, b)], g[b][0], om);
+                            ok = fma(kxs[tri<P>(axo, b)], g[b][KG], ok);
+                            ok = fma(mxs[tri<P>(axo, b)], g[b][1], ok);
+                        }
+                        l2 = fma(cs[slot[azo]][wv + ayo][lane + axo], om, l2);
+                        if constexpr (GRAD) gacc = fma(ns[slot[azo]][wv + ayo][lane + axo], ok, gacc);
+                        // lanes of one wave address distinct columns here; the
+                        // LDS applies one wave's adds in program order
+                        atomic_add(&fs[slot[azo]][wv + ayo][lane + axo], A(fma(s, ok, om)));
+                    }
+                }
+            }
+            if constexpr (GRAD)
+                if (active) dkappa[eidx] += double(ndt * gacc);
+        }
+        __syncthreads();
+        flush_plane(ez);
+    }
     for (int z = ez1; z < ez1 + KL; ++z) flush_plane(z);
 
     const double l2d = wave_sum(double(l2));
@@ -779,14 +1097,19 @@ AsmDims base_dims(const Tables<T>& tb)
     return dm;
 }
 
+#ifndef ADS_HEAT_DPPR
+#define ADS_HEAT_DPPR 0
+#endif
+
 template <typename T, int P, int MODE>
 hipError_t heat_p(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
                   int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
 {
+    constexpr bool DPPR = ADS_HEAT_DPPR && MODE == kHeatPlain;
     // two rounds of resident workgroups (a partly filled last round costs
     // as much as a full one)
     static const int resident =
-        resident_groups(reinterpret_cast<const void*>(&k_heat<T, P, MODE>), kWave * kRowsWg, 0);
+        resident_groups(reinterpret_cast<const void*>(&k_heat<T, P, MODE, DPPR>), kWave * kRowsWg, 0);
     AsmDims dm = base_dims(tb);
     dm.yblocks = (dm.ney + kRowsWg - 1) / kRowsWg;
     dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.yblocks, dm.nez, 2L * resident);
@@ -794,9 +1117,43 @@ hipError_t heat_p(const double* c, T* F, const Tables<T>& tb, double dt, const d
     dim3 grid(dm.xchunks * dm.yblocks * nseg);
     *nparts = static_cast<int>(grid.x);
     if (*nparts > part_cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_heat<T, P, MODE>), grid, dim3(kWave * kRowsWg), 0, s, c, F, tb.d[0].emat,
+    hipLaunchKernelGGL((k_heat<T, P, MODE, DPPR>), grid, dim3(kWave * kRowsWg), 0, s, c, F, tb.d[0].emat,
                        tb.d[1].emat_sym, tb.d[2].emat_sym, kappa, static_cast<T>(-dt), dm, l2_part, cn, dkappa);
     return hipGetLastError();
+}
+
+template <typename T, int P, int MODE>
+hipError_t heat_pf_p(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
+                     int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
+{
+    static const int resident =
+        resident_groups(reinterpret_cast<const void*>(&k_heat_pf<T, P, MODE>), kWave * kRowsWg, 0);
+    AsmDims dm = base_dims(tb);
+    dm.yblocks = (dm.ney + kRowsWg - 1) / kRowsWg;
+    dm.zseg = segment_length(static_cast<long>(dm.xchunks) * dm.yblocks, dm.nez, 2L * resident);
+    const int nseg = (dm.nez + dm.zseg - 1) / dm.zseg;
+    dim3 grid(dm.xchunks * dm.yblocks * nseg);
+    *nparts = static_cast<int>(grid.x);
+    if (*nparts > part_cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_heat_pf<T, P, MODE>), grid, dim3(kWave * kRowsWg), 0, s, c, F, tb.d[0].emat_sym,
+                       tb.d[1].emat_sym, tb.d[2].emat_sym, kappa, static_cast<T>(-dt), dm, l2_part, cn, dkappa);
+    return hipGetLastError();
+}
+
+#ifndef ADS_HEAT_PF
+#define ADS_HEAT_PF 1
+#endif
+
+// the heat assembly: the position-first kernel (k_heat_pf), or with
+// -DADS_HEAT_PF=0 at build time the element-first one (k_heat, whose
+// -DADS_HEAT_DPPR=1 instance pre-reduces the x offsets by DPP); measured at
+// 256^3 elements, p = 3: 1.20, 1.65 and 2.26 ms on one box
+template <typename T, int P, int MODE>
+hipError_t heat_sel(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
+                    int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
+{
+    if (ADS_HEAT_PF) return heat_pf_p<T, P, MODE>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, cn, dkappa, s);
+    return heat_p<T, P, MODE>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, cn, dkappa, s);
 }
 
 template <typename T, int P>
@@ -820,9 +1177,8 @@ template <typename T, int P>
 hipError_t assemble_p(int form, int f0, const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa,
                       double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
-    return form == kFormHeat
-               ? heat_p<T, P, kHeatPlain>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, nullptr, nullptr, s)
-               : project_p<T, P>(F, tb, f0, l2_part, part_cap, nparts, s);
+    if (form != kFormHeat) return project_p<T, P>(F, tb, f0, l2_part, part_cap, nparts, s);
+    return heat_sel<T, P, kHeatPlain>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, nullptr, nullptr, s);
 }
 
 template <int P>
@@ -931,10 +1287,10 @@ hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, T* la
     if (!dkappa)
         return launch_assemble<T>(p, kFormHeat, 0, mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, s);
     switch (p) {
-    case 1: return heat_p<T, 2, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 2: return heat_p<T, 3, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 3: return heat_p<T, 4, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
-    case 4: return heat_p<T, 5, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 1: return heat_sel<T, 2, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 2: return heat_sel<T, 3, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 3: return heat_sel<T, 4, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 4: return heat_sel<T, 5, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -943,10 +1299,10 @@ hipError_t launch_heat_dual(int p, const double* c_val, const double* c_grad, do
                             double dt, const double* kappa, double* l2_part, int part_cap, int* nparts, hipStream_t s)
 {
     switch (p) {
-    case 1: return heat_p<double, 2, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
-    case 2: return heat_p<double, 3, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
-    case 3: return heat_p<double, 4, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
-    case 4: return heat_p<double, 5, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 1: return heat_sel<double, 2, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 2: return heat_sel<double, 3, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 3: return heat_sel<double, 4, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 4: return heat_sel<double, 5, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -34,10 +34,12 @@ runs on, bracketed by device synchronisation).  Alongside it:
     its minimum traffic, gradient/forward, errors of the state and both
     gradients);
   * ``roofline``: the heat assembly kernel (the step's dominant kernel) timed
-    alone with HIP events on the library stream (ads_time_kernel), its
-    algorithmic FP64 flops (element-operator form; the point-quadrature count
-    beside it) against MI355X's FP64 peak; ``kron_residual`` the same for the
-    fused residual kernel of the fast precision form, by bytes;
+    alone with HIP events on the library stream (ads_time_kernel), the FP64
+    flops it executes (position-first form) against MI355X's FP64 peak, with
+    the equivalent rates of the element-operator and point-quadrature counts
+    beside it; ``solves`` the three FP64/FP32 mass solves by bytes (16 B/DOF
+    per direction in FP64); ``kron_residual`` the fused residual kernel of the
+    fast precision form, by bytes;
   * ``oil``: the oil problem (problems/oil: kq(X) exp(mi u) permeability,
     pumps, drains) at 128^3 elements, p = 2: step time, its point-quadrature
     kernel against the FP64 peak, and the oracle's faithful step on a 48^3
@@ -69,10 +71,19 @@ PEAK_HBM_GBS = 8000.0
 
 
 def heat_kernel_flops(E, P):
-    """Algorithmic flops of one heat-assembly launch: per element 8 P^4 FMAs
-    (x stage 2 P^4, z stage 3 P^4, y stage 3 P^4 for M c and K c of the
-    element operator) and 2 P^3 FMAs (combining M c - dt kappa K c, and
-    c . M c for l2): 16 P^4 + 4 P^3 flops."""
+    """Flops of one launch of the position-first heat assembly (k_heat_pf)
+    per element: the y and z contractions once per x position, 5 P^3 FMA
+    (z: 2 P^3 for Mz c and Kz c; y: 3 P^3 for My, Ky of those); the x
+    contractions over the element's P positions, 3 P^4 FMA (Mx GM, Kx GM,
+    Mx GS); M c - dt kappa K c and c . M c, 2 P^3 FMA: 6 P^4 + 14 P^3
+    flops (2,432 at p = 3).  The halo positions' share is not counted."""
+    return E * (6 * P ** 4 + 14 * P ** 3)
+
+
+def element_operator_flops(E, P):
+    """Flops of the same operator applied element by element (x, z, y
+    stages per element and x test index, 8 P^4 + 2 P^3 FMA): 16 P^4 + 4 P^3
+    per element (4,352 at p = 3)."""
     return E * (16 * P ** 4 + 4 * P ** 3)
 
 
@@ -318,8 +329,8 @@ def main():
     solves = {}
     for key, which, bpd, what in (
             ("fp64_partitioned", _lib.ADS_KERNEL_SOLVE_FP64, 48,
-             "k_part_rows + 2 x k_part_lines<double,3> (no-interchange LU, lines in partitions of <= 32 "
-             "solved in parallel, exact edge-state scans)"),
+             "k_part_rows + 2 x k_part_lines<double,3> (no-interchange LU, lines in partitions of 24 "
+             "solved in parallel, exact edge-state scans; persistent, the next group loaded meanwhile)"),
             ("fp64_lu", _lib.ADS_KERNEL_SOLVE_FP64_LU, 48,
              "k_sweep_rows + 2 x k_sweep_lines<double,3> (dgbtrf/dgbtrs form, one lane per line)"),
             ("fp32_segmented", _lib.ADS_KERNEL_SOLVE_SEG_FP32, 24,
@@ -503,13 +514,22 @@ def main():
         "oil": oil,
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": "k_heat<double,P=4> (FP64 VALU; MI355X's FP64 vector and matrix peaks are equal)",
-                     "basis": ("element-operator form: per element F_e = (Mz(x)My(x)Mx - dt kappa (Mz(x)My(x)Kx + "
-                               "Mz(x)Ky(x)Mx + Kz(x)My(x)Mx)) c_e with the 1-D element matrices, 8P^4 + 2P^3 FMA = "
-                               "16P^4 + 4P^3 flops per element (equal to the Gauss-point sum: Q = p+1 points "
-                               "integrate the degree-2p integrands exactly)"),
+                     "kernel": "k_heat_pf<double,P=4,plain> (FP64 VALU; MI355X's FP64 vector and matrix peaks are "
+                               "equal)",
+                     "basis": ("flops the kernel executes per element: the element operator F_e = (Mz(x)My(x)Mx - "
+                               "dt kappa (Mz(x)My(x)Kx + Mz(x)Ky(x)Mx + Kz(x)My(x)Mx)) c_e with the 1-D element "
+                               "matrices, its y and z contractions applied once per x position and shared by the "
+                               "P elements there: 3P^4 + 7P^3 FMA = 6P^4 + 14P^3 flops per element (equal to the "
+                               "Gauss-point sum: Q = p+1 points integrate the degree-2p integrands exactly)"),
                      "ms_per_launch": ms_asm, "flops_per_launch": flops,
-                     "flops_per_element": 16 * P ** 4 + 4 * P ** 3, "elements_per_launch": E,
+                     "flops_per_element": 6 * P ** 4 + 14 * P ** 3, "elements_per_launch": E,
+                     "element_operator_form": {
+                         "flops_per_element": 16 * P ** 4 + 4 * P ** 3,
+                         "note": ("the same operator applied element by element (8P^4 + 2P^3 FMA), which the "
+                                  "element-first kernel k_heat executes; equivalent rate of this launch"),
+                         "equivalent_achieved": element_operator_flops(E, P) / (ms_asm * 1e-3) / 1e12,
+                         "equivalent_frac": element_operator_flops(E, P) / (ms_asm * 1e-3) / 1e12
+                         / PEAK_FP64_TFLOPS},
                      "quadrature_form": {
                          "flops_per_element": 36 * P ** 4 + 12 * P ** 3,
                          "note": ("the sum-factorised point-quadrature count (u and grad u at the Q^3 points, "

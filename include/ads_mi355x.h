@@ -14,6 +14,13 @@
  * "_d" entry points take device pointers and run on the context's stream
  * (ads_set_stream); they do not synchronise.  Host-pointer entry points copy
  * and synchronise.  Calls on one context are not reentrant.
+ *
+ * Device buffers must be ordinary (coarse-grained) device memory: hipMalloc,
+ * or a torch tensor on the GPU.  The assembly adds into its output with
+ * floating-point global atomics (built with -munsafe-fp-atomics), which are
+ * not guaranteed on fine-grained memory (hipHostMalloc, coherent
+ * hipMallocManaged, fine-grained hipExtMallocWithFlags): results there may
+ * be wrong without an error.
  */
 #ifndef ADS_MI355X_H
 #define ADS_MI355X_H
@@ -69,7 +76,9 @@ typedef struct ads_ctx ads_ctx;
 
 /* ADSS::initialize(n, p, ads, ads_data, mierr) -- src/ADS.F90:11-107.
  * n = number of basis functions minus one per direction (the heat CLI SIZE),
- * p = B-spline degree per direction (1..4, equal in all directions).
+ * p = B-spline degree per direction (1..4 each; directions may differ, as
+ * in the reference's p(3): the kernels run at the largest degree on tables
+ * widened with exact zeros).
  * Builds the knots (PrepareKnot, knot_vector.F90:43-61), the basis tables
  * (BasisData, basis.F90:74-112), the 1-D mass and stiffness matrices
  * (MKBBT, projection_engine.F90:63-132) and their FP64 banded LU
@@ -202,7 +211,8 @@ int ads_last_drained(ads_ctx* ctx, double* drained);
 /* The algorithm of the FP64 mass solves (ADS_SOLVER_*; no reference
  * counterpart: the reference's MUMPS solves one column at a time).  AUTO
  * (the default) takes the partitioned solve when every direction's lines
- * have at most 512 entries; PARTITIONED is refused beyond that. */
+ * have at most 384 entries (16 partitions of 24) and an x-y plane stays
+ * below 4 GiB; PARTITIONED is refused beyond that. */
 int ads_set_solver(ads_ctx* ctx, int solver);
 
 /* Form3DRHS after FormUn (projection_engine.F90:154-284) on the context's
