@@ -783,9 +783,45 @@ __device__ __forceinline__ double oil_falloff(double r, double R, double t)
     return g * g;
 }
 
+// Segment s = 0 .. cN (cL - 1) - 1 of the curves: its end point index k
+// (points k - 1 and k).
+__device__ __forceinline__ int oil_seg_end(const OilDev& o, int s)
+{
+    return (s / (o.cL - 1)) * o.cL + s % (o.cL - 1) + 1;
+}
+
+// Whether segment (k - 1, k) can lie within sqrt(cut2) of a point of the box
+// [lo, hi]: the distance between the boxes is a lower bound of the
+// segment's distance to any point of the box.
+__device__ __forceinline__ bool oil_seg_near(const OilDev& o, int k, const double (&lo)[3], const double (&hi)[3],
+                                             double cut2)
+{
+    const int npt = o.cN * o.cL;
+    double d2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double u = o.curves[a * npt + k - 1], v = o.curves[a * npt + k];
+        const double gap = fmax(0.0, fmax(lo[a] - fmax(u, v), fmin(u, v) - hi[a]));
+        d2 += gap * gap;
+    }
+    return d2 <= cut2;
+}
+
+// The candidate segments of a falloff of radius R: within R (with a margin
+// for rounding) of the box.  Every segment whose computed distance to a
+// point of the box is below R^2 (1 + 1e-9) is listed, so the minimum over
+// the list equals the minimum over all segments wherever falloff(0, R, .)
+// of it is not 0, and both give 0 elsewhere: the results are identical.
+__device__ __forceinline__ double oil_cut2(double R)
+{
+    return R * R * (1.0 + 1e-9);
+}
+
 // squared distance from the curves (dist_from_curves / dist_from_segment,
-// input_data.F90:171-238), at most 1e3
-__device__ double oil_dist_curves(const OilDev& o, double px, double py, double pz)
+// input_data.F90:171-238), at most 1e3, over the segments ends[0 .. nlist)
+// (their order does not matter: the minimum is exact)
+__device__ double oil_dist_curves(const OilDev& o, const int* __restrict__ ends, int nlist, double px, double py,
+                                  double pz)
 {
 #pragma clang fp contract(off)
     const int npt = o.cN * o.cL;
@@ -793,27 +829,26 @@ __device__ double oil_dist_curves(const OilDev& o, double px, double py, double 
     const double* __restrict__ cy = o.curves + npt;
     const double* __restrict__ cz = o.curves + 2 * npt;
     double fval = 1e3;
-    for (int i = 0; i < o.cN; ++i)
-        for (int j = 1; j < o.cL; ++j) {
-            const int k = i * o.cL + j;
-            const double ax = cx[k - 1], ay = cy[k - 1], az = cz[k - 1];
-            const double dx = cx[k] - ax, dy = cy[k] - ay, dz = cz[k] - az;
-            const double qx = px - ax, qy = py - ay, qz = pz - az;
-            const double dot = dx * qx + dy * qy + dz * qz;
-            const double len2 = dx * dx + dy * dy + dz * dz;
-            const double proj = dot / len2;
-            double xx, yy, zz;
-            if (proj < 0) {
-                xx = ax; yy = ay; zz = az;
-            } else if (proj > 1) {
-                xx = cx[k]; yy = cy[k]; zz = cz[k];
-            } else {
-                xx = ax + proj * dx; yy = ay + proj * dy; zz = az + proj * dz;
-            }
-            const double ex = px - xx, ey = py - yy, ez = pz - zz;
-            const double d = ex * ex + ey * ey + ez * ez;
-            if (d < fval) fval = d;
+    for (int s = 0; s < nlist; ++s) {
+        const int k = ends[s];
+        const double ax = cx[k - 1], ay = cy[k - 1], az = cz[k - 1];
+        const double dx = cx[k] - ax, dy = cy[k] - ay, dz = cz[k] - az;
+        const double qx = px - ax, qy = py - ay, qz = pz - az;
+        const double dot = dx * qx + dy * qy + dz * qz;
+        const double len2 = dx * dx + dy * dy + dz * dz;
+        const double proj = dot / len2;
+        double xx, yy, zz;
+        if (proj < 0) {
+            xx = ax; yy = ay; zz = az;
+        } else if (proj > 1) {
+            xx = cx[k]; yy = cy[k]; zz = cz[k];
+        } else {
+            xx = ax + proj * dx; yy = ay + proj * dy; zz = az + proj * dz;
         }
+        const double ex = px - xx, ey = py - yy, ez = pz - zz;
+        const double d = ex * ex + ey * ey + ez * ez;
+        if (d < fval) fval = d;
+    }
     return fval;
 }
 
@@ -836,32 +871,75 @@ __device__ __forceinline__ double oil_sources(const double* __restrict__ q, int 
     return f;
 }
 
-// initial_state (input_data.F90:317-329): 0.1 lerp(falloff(0, .1, dist), 0, 1) bump3d(.2, .6, X)
-__device__ double oil_initial(const OilDev& o, double x, double y, double z)
+// initial_state (input_data.F90:317-329): 0.1 lerp(falloff(0, .1, dist), 0, 1) bump3d(.2, .6, X),
+// the distance over the candidate segments of radius 0.1
+__device__ double oil_initial(const OilDev& o, const int* __restrict__ ends, int nlist, double x, double y, double z)
 {
 #pragma clang fp contract(off)
-    const double dist = sqrt(oil_dist_curves(o, x, y, z));
+    const double dist = sqrt(oil_dist_curves(o, ends, nlist, x, y, z));
     const double bx = x - 0.5, by = y - 0.5, bz = z - 0.5;
     const double b = oil_falloff(0.2 / 2.0, 0.6 / 2.0, sqrt(bx * bx + by * by + bz * bz));
     const double t = oil_falloff(0.0, 0.1, dist);
     return 0.1 * ((1.0 - t) * 0.0 + t * 1.0) * b;
 }
 
+// One thread per Gauss point.  The block lists the candidate segments of
+// radius 0.06 (the falloff of kq) for the box of its points first (in LDS,
+// cN (cL - 1) entries at most), so a point scans a few segments instead of
+// all of them.
 template <int Q>
 __global__ __launch_bounds__(256) void k_oil_kq(const double* __restrict__ ptsx, const double* __restrict__ ptsy,
                                                 const double* __restrict__ ptsz, int nex, int ney, int nez, OilDev o,
                                                 double* __restrict__ kq)
 {
 #pragma clang fp contract(off)
-    const long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    extern __shared__ int ends[];
+    __shared__ double wlo[4][3], whi[4][3];
+    __shared__ int nlist;
     const long n = static_cast<long>(nex) * ney * nez * Q * Q * Q;
-    if (i >= n) return;
+    const long i0 = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+    const long i = i0 < n ? i0 : n - 1;  // past the end: the last point (not stored)
     const int kx = static_cast<int>(i % Q), ky = static_cast<int>((i / Q) % Q), kz = static_cast<int>((i / (Q * Q)) % Q);
     const long e = i / (Q * Q * Q);
     const int ex = static_cast<int>(e % nex), ey = static_cast<int>((e / nex) % ney), ez = static_cast<int>(e / (static_cast<long>(nex) * ney));
-    const double dist = sqrt(oil_dist_curves(o, ptsx[ex * Q + kx], ptsy[ey * Q + ky], ptsz[ez * Q + kz]));
+    const double pt[3] = {ptsx[ex * Q + kx], ptsy[ey * Q + ky], ptsz[ez * Q + kz]};
+    // the block's bounding box: per wave by shuffles, then across the waves
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (threadIdx.x == 0) nlist = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double lo = pt[a], hi = pt[a];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, off, kWave));
+            hi = fmax(hi, __shfl_xor(hi, off, kWave));
+        }
+        if (lane == 0) {
+            wlo[w][a] = lo;
+            whi[w][a] = hi;
+        }
+    }
+    __syncthreads();
+    double lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wlo[0][a];
+        hi[a] = whi[0][a];
+        for (int v = 1; v < static_cast<int>(blockDim.x / kWave); ++v) {
+            lo[a] = fmin(lo[a], wlo[v][a]);
+            hi[a] = fmax(hi[a], whi[v][a]);
+        }
+    }
+    const int nseg = o.cN * (o.cL - 1);
+    const double cut2 = oil_cut2(0.06);
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+        const int k = oil_seg_end(o, s);
+        if (oil_seg_near(o, k, lo, hi, cut2)) ends[atomicAdd(&nlist, 1)] = k;
+    }
+    __syncthreads();
+    const double dist = sqrt(oil_dist_curves(o, ends, nlist, pt[0], pt[1], pt[2]));
     const double t = oil_falloff(0.0, 0.06, dist);
-    kq[i] = (1.0 - t) * o.kq_min + t * o.kq_max;
+    if (i0 < n) kq[i0] = (1.0 - t) * o.kq_min + t * o.kq_max;
 }
 
 template <int P, bool TZERO>
@@ -892,6 +970,10 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
     const int sx = dm.sx;
     const long plane = static_cast<long>(sx) * dm.sy;
     const int ncol = min(W, sx - ex0);
+    // TZERO: the candidate segments of radius 0.1 (initial_state's falloff)
+    // for the current element layer's points (cN (cL - 1) entries at most)
+    extern __shared__ int ends[];
+    int nlist = 0;
 
     const double* __restrict__ Nx = tabx + static_cast<size_t>(exc) * Q * P * 2;
     const double* __restrict__ Ny = taby + static_cast<size_t>(ey) * Q * P * 2;
@@ -918,6 +1000,27 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
 #pragma unroll
         for (int az = 0; az < P; ++az) slot[az] = (s0 + az) % P;
         const long e = exc + static_cast<long>(dm.nex) * (ey + static_cast<long>(dm.ney) * ez);
+        if constexpr (TZERO) {
+            // the layer's points span x of elements ex0 .. ex0+63 and y, z of (ey, ez)
+            const int exl = min(ex0 + kWave - 1, dm.nex - 1);
+            const double lo[3] = {ptsx[static_cast<size_t>(ex0) * Q], ptsy[static_cast<size_t>(ey) * Q],
+                                  ptsz[static_cast<size_t>(ez) * Q]};
+            const double hi[3] = {ptsx[static_cast<size_t>(exl) * Q + Q - 1], ptsy[static_cast<size_t>(ey) * Q + Q - 1],
+                                  ptsz[static_cast<size_t>(ez) * Q + Q - 1]};
+            const int nseg = o.cN * (o.cL - 1);
+            const double cut2 = oil_cut2(0.1);
+            __syncthreads();  // the previous layer's readers are done
+            nlist = 0;
+            for (int s0 = 0; s0 < nseg; s0 += kWave) {
+                const int s = s0 + lane;
+                const int k = s < nseg ? oil_seg_end(o, s) : 0;
+                const bool near = s < nseg && oil_seg_near(o, k, lo, hi, cut2);
+                const unsigned long long m = __ballot(near);
+                if (near) ends[nlist + __popcll(m & ((1ull << lane) - 1))] = k;
+                nlist += __popcll(m);
+            }
+            __syncthreads();
+        }
         // the element's coefficients: first DOF of element e is e (open knots)
         const double* __restrict__ ce = c ? c + exc + static_cast<long>(sx) * ey + plane * ez : nullptr;
 
@@ -995,7 +1098,7 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
                     const double zq = ptsz[static_cast<size_t>(ez) * Q + qz];
                     double w0, gx = 0.0, gy = 0.0, gz = 0.0;
                     if constexpr (TZERO) {
-                        const double f0 = oil_initial(o, xq, yq, zq);
+                        const double f0 = oil_initial(o, ends, nlist, xq, yq, zq);
                         w0 = w * f0;
                         l2 = fma(w0, f0, l2);
                     } else {
@@ -1192,8 +1295,9 @@ hipError_t oil_p(bool tzero, const double* c, double* F, const Tables<double>& t
     dim3 grid(dm.xchunks * dm.ney * nseg);
     *nparts = static_cast<int>(grid.x);
     if (*nparts > part_cap) return hipErrorInvalidValue;
+    const size_t list_bytes = sizeof(int) * static_cast<size_t>(o.cN) * (o.cL - 1);
     if (tzero)
-        hipLaunchKernelGGL((k_oil<P, true>), grid, dim3(kWave), 0, s, c, F, tb.d[0].tab, tb.d[1].tab, tb.d[2].tab,
+        hipLaunchKernelGGL((k_oil<P, true>), grid, dim3(kWave), list_bytes, s, c, F, tb.d[0].tab, tb.d[1].tab, tb.d[2].tab,
                            tb.d[0].jac, tb.d[1].jac, tb.d[2].jac, tb.d[0].wts, tb.d[1].wts, tb.d[2].wts, tb.d[0].pts,
                            tb.d[1].pts, tb.d[2].pts, dm, o, kq, dt, l2_part, dr_part);
     else
@@ -1318,11 +1422,12 @@ hipError_t launch_oil_kq(int p, const Tables<double>& tb, const OilDev& o, doubl
     const long q3 = static_cast<long>(p + 1) * (p + 1) * (p + 1);
     const long n = static_cast<long>(nex) * ney * nez * q3;
     const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+    const size_t list_bytes = sizeof(int) * static_cast<size_t>(o.cN) * (o.cL - 1);
     switch (p) {
-    case 1: hipLaunchKernelGGL((k_oil_kq<2>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
-    case 2: hipLaunchKernelGGL((k_oil_kq<3>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
-    case 3: hipLaunchKernelGGL((k_oil_kq<4>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
-    case 4: hipLaunchKernelGGL((k_oil_kq<5>), grid, dim3(256), 0, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 1: hipLaunchKernelGGL((k_oil_kq<2>), grid, dim3(256), list_bytes, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 2: hipLaunchKernelGGL((k_oil_kq<3>), grid, dim3(256), list_bytes, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 3: hipLaunchKernelGGL((k_oil_kq<4>), grid, dim3(256), list_bytes, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
+    case 4: hipLaunchKernelGGL((k_oil_kq<5>), grid, dim3(256), list_bytes, s, tb.d[0].pts, tb.d[1].pts, tb.d[2].pts, nex, ney, nez, o, kq); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -99,6 +99,43 @@ def test_oil_run_vs_oracle(orc, n, p, steps):
         adss.Cleanup(ads, data)
 
 
+def test_oil_64_cubed_realistic_inputs(orc):
+    """The oil problem at 64^3 elements, p = 2, with the benchmark's inputs
+    (30 random curves of 16 points, seed 1; two pumps, two drains; mi = 10,
+    Kq 1..1000, radius 0.15, dt = 1e-9): the kq cache bit-exact, the
+    projection and two steps (coefficients, l2norm, drained) within 1e-12."""
+    n, p, dt = (65, 65, 65), 2, 1e-9
+    say("case oil 64^3")
+    S = orc.Setup(n, p)
+    kw = dict(pumps=[(0.3, 0.3, 0.5), (0.7, 0.7, 0.5)], drains=[(0.5, 0.5, 0.2), (0.5, 0.5, 0.8)], mi=10.0,
+              kq_min=1.0, kq_max=1000.0, radius=0.15, pump_strength=1.0, drain_strength=1.0)
+    oil = orc.Oil.random(1, cN=30, cL=16, **kw)
+    rhs = adss.OilRHS(dt, oil.cx, oil.cy, oil.cz, 30, 16, **kw)
+    kq_ref = S.oil_kq_cache(oil)
+    ads, data = adss.initialize(n, p)
+    try:
+        l2 = adss.Step(0, rhs, ads, data)
+        L = _lib.load()
+        kq = np.zeros(S.E * (p + 1) ** 3)
+        _lib.check(L.ads_oil_get_kq(data.handle, kq.ctypes.data_as(adss._dp)), data.handle)
+        assert kq_ref.max() > kq_ref.min()
+        assert np.array_equal(kq, kq_ref)
+        c, l2o = S.oil_project(oil)
+        say("projection rel err", rel(data.F, c))
+        assert l2o > 0 and rel(data.F, c) <= TOL and abs(l2 - l2o) <= TOL * l2o
+        dro_tot = 0.0
+        for it in (1, 2):
+            rhs.t += dt
+            l2 = adss.Step(it, rhs, ads, data)
+            c, l2o, dro = S.oil_step(oil, kq_ref, c, dt)
+            dro_tot += dro
+            say("step", it, "rel err", rel(data.F, c), "drained", rhs.drained, dro_tot)
+            assert rel(data.F, c) <= TOL and abs(l2 - l2o) <= TOL * l2o
+            assert dro_tot > 0 and abs(rhs.drained - dro_tot) <= TOL * dro_tot
+    finally:
+        adss.Cleanup(ads, data)
+
+
 def test_oil_entry_points_reject_misuse():
     ads, data = adss.initialize(9, 2)
     L = _lib.load()
