@@ -40,6 +40,9 @@ runs on, bracketed by device synchronisation).  Alongside it:
     beside it; ``solves`` the three FP64/FP32 mass solves by bytes (16 B/DOF
     per direction in FP64); ``kron_residual`` the fused residual kernel of the
     fast precision form, by bytes;
+  * ``shard``: one GPU's share of a z-slab decomposition over 8 GPUs
+    (256x256x32 elements): the step and its kernels measured at that shape,
+    with the all-to-all and halo traffic modelled (not measured);
   * ``oil``: the oil problem (problems/oil: kq(X) exp(mi u) permeability,
     pumps, drains) at 128^3 elements, p = 2: step time, its point-quadrature
     kernel against the FP64 peak, and the oracle's faithful step on a 48^3
@@ -219,6 +222,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-oil", action="store_true")
+    ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: no CPU baseline, no gradient, plus one cold 1 GiB device "
                          "copy (known bytes) to calibrate FETCH_SIZE/WRITE_SIZE")
@@ -226,6 +230,7 @@ def main():
     if args.profile:
         args.no_cpu_baseline = True
         args.no_grad = True
+        args.no_shard = True
         args.steps = min(args.steps, 5)
         args.warmup = min(args.warmup, 1)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -445,6 +450,52 @@ def main():
         del runs
         solver5.close()
 
+    # ---- one GPU's share of a z-slab decomposition over 8 GPUs ------------
+    # 256 x 256 x 32 elements (259 x 259 x 35 DOFs with its p halo planes):
+    # the heat assembly and the x and y solves at their real shape; the z
+    # solve here runs on 35-entry lines (after the all-to-all of a real
+    # decomposition it would run on 259-entry lines of a y slab, the same
+    # bytes per DOF).  Not a multi-GPU measurement: the communication is a
+    # model (all-to-all over 7 xGMI links, one neighbour halo).
+    shard = None
+    if not args.no_shard:
+        G = 8
+        ads_s, data_s = adss.initialize((n, n, (n - p + 1) // G + p - 1), p)
+        hs = data_s.handle
+        L.ads_set_stream(hs, ctypes.c_void_p(stream.cuda_stream))
+        data_s.F = adss.uniform(7, data_s.D, -1.0, 1.0)
+        for _ in range(args.warmup):
+            _lib.check(L.ads_step(hs, dt, _lib.ADS_PREC_FP64, 0, None), hs)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            _lib.check(L.ads_step(hs, dt, _lib.ADS_PREC_FP64, 0, None), hs)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_s = e0.elapsed_time(e1) / args.steps
+        _lib.check(L.ads_time_kernel(hs, _lib.ADS_KERNEL_HEAT_FP64, dt, 2, ctypes.byref(ms_k)), hs)
+        _lib.check(L.ads_time_kernel(hs, _lib.ADS_KERNEL_HEAT_FP64, dt, args.steps, ctypes.byref(ms_k)), hs)
+        ms_s_asm = ms_k.value
+        _lib.check(L.ads_time_kernel(hs, _lib.ADS_KERNEL_SOLVE_FP64, dt, 2, ctypes.byref(ms_k)), hs)
+        _lib.check(L.ads_time_kernel(hs, _lib.ADS_KERNEL_SOLVE_FP64, dt, args.steps, ctypes.byref(ms_k)), hs)
+        ms_s_solve = ms_k.value
+        # model: the z <-> y all-to-all before and after the z solve (each
+        # GPU sends (G-1)/G of its D/G doubles over 7 links of 153 GB/s) and
+        # the p-plane halo to one neighbour (c planes out, partial F planes
+        # back) on one link, not overlapped
+        a2a_ms = 2 * 8.0 * D / G * (G - 1) / G / (7 * 153e9) * 1e3
+        halo_ms = 2 * 8.0 * p * (n + 1) ** 2 / 153e9 * 1e3
+        t8 = ms_s + a2a_ms + halo_ms
+        shard = {"workload": "one z slab of %d: 256x256x32 elements, p=3, %dx%dx%d DOFs (with its p halo planes)"
+                             % (G, n + 1, n + 1, data_s.D // (n + 1) ** 2),
+                 "gpus_modelled": G, "ms_per_step": ms_s, "heat_kernel_ms": ms_s_asm, "solve_fp64_ms": ms_s_solve,
+                 "vs_full_step_over_G": ms_s / (ms64 / G),
+                 "model": {"all_to_all_ms": a2a_ms, "halo_ms": halo_ms, "step_ms": t8,
+                           "value": D / (t8 * 1e-3), "unit": "DOF/s",
+                           "note": "compute measured on one GPU at the slab's shape; communication modelled, "
+                                   "not measured (one GPU per box)"}}
+        adss.Cleanup(ads_s, data_s)
+
     # ---- oil problem at 128^3, p = 2 ---------------------------------------
     oil = None
     if not args.no_oil:
@@ -511,6 +562,7 @@ def main():
         "fp64_kron": forms["fp64_kron"],
         "grad_fwd": grad,
         "config5": cfg5,
+        "shard": shard,
         "oil": oil,
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
