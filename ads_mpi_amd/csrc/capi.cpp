@@ -262,16 +262,19 @@ int seg_solve(ads_ctx* ctx, const float* in, float* tmp, float* out, const ads::
 }
 
 // The device tables of the partitioned solve for every direction, when the
-// lines fit its kernels (at most kPartMaxK partitions of kPartSeg entries).
+// lines fit its kernels (at most kPartMaxK partitions of kPartSeg entries, or
+// of kPartSegLong for lines longer than kPartMaxK * kPartSeg).
 int build_part(ads_ctx* ctx)
 {
     ctx->part_ok = false;
     if (!ads::part_planes_fit(ctx->kdir[0].dofs, ctx->kdir[1].dofs, sizeof(double))) return ADS_OK;  // LU sweeps
     for (int d = 0; d < 3; ++d) {
+        const int seg = ads::part_seg(ctx->kdir[d].dofs);
+        if (seg == 0) return ADS_OK;  // the LU sweeps serve these lines
         ads::PartHost ph;
-        const std::string e = ads::part_tables(ctx->kdir[d], ads::kPartSeg, ph);
+        const std::string e = ads::part_tables(ctx->kdir[d], seg, ph);
         if (!e.empty()) return fail(ctx, ADS_ERR_SINGULAR, "direction " + std::to_string(d) + ": " + e);
-        if (ph.K > ads::kPartMaxK) return ADS_OK;  // the LU sweeps serve these lines
+        if (ph.K > ads::kPartMaxK) return ADS_OK;
         double *lr = nullptr, *ur = nullptr, *gs = nullptr, *hs = nullptr;
         int rc;
         if ((rc = upload(ctx, &lr, ph.lrow)) || (rc = upload(ctx, &ur, ph.urow)) || (rc = upload(ctx, &gs, ph.gsp)) ||
@@ -1407,7 +1410,7 @@ int ads_set_solver(ads_ctx* ctx, int solver)
     if (solver == ADS_SOLVER_PARTITIONED && !ctx->part_ok)
         return fail(ctx, ADS_ERR_UNSUPPORTED,
                     "ads_set_solver: the partitioned solve takes lines of at most " +
-                        std::to_string(ads::kPartMaxK * ads::kPartSeg) + " entries");
+                        std::to_string(ads::kPartMaxLine) + " entries");
     ctx->solver = solver;
     return ADS_OK;
 }
@@ -1418,8 +1421,12 @@ int ads_host_part_tables(int n, int p, int* K, int* S, double* lrow, double* uro
     ads::Direction d;
     std::string e = ads::build_direction(n, p, d);
     if (!e.empty()) return fail(nullptr, ADS_ERR_INVALID, e);
+    const int seg = ads::part_seg(d.dofs);
+    if (seg == 0)
+        return fail(nullptr, ADS_ERR_UNSUPPORTED,
+                    "ads_host_part_tables: lines of at most " + std::to_string(ads::kPartMaxLine) + " entries");
     ads::PartHost ph;
-    e = ads::part_tables(d, ads::kPartSeg, ph);
+    e = ads::part_tables(d, seg, ph);
     if (!e.empty()) return fail(nullptr, ADS_ERR_SINGULAR, e);
     *K = ph.K;
     *S = ph.S;

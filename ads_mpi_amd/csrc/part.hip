@@ -9,6 +9,8 @@
 // workgroups are persistent and load the next group while they solve the
 // current one; a wave keeps its partition's factor rows spread over its
 // lanes for the whole launch.
+#include <type_traits>
+
 #include "part.hpp"
 
 namespace ads {
@@ -472,44 +474,67 @@ hipError_t persistent_grid(KERNEL kernel, int block, size_t lds, int ngroups, in
     return hipSuccess;
 }
 
-template <typename T, int KL, int KMAX, int TILE>
-hipError_t part_launch(T* F, const PartDev<T> pd[3], hipStream_t s)
+// One direction's launch, the instance chosen by its partition length and
+// count: double-buffered up to kPipeMaxK partitions of kPartSeg, one group at
+// a time beyond (kPartMaxK partitions of kPartSeg or kPartSegLong).
+template <typename T, int KL, int SM, int KMAX, int TILE>
+hipError_t part_rows_launch(T* F, int rows, const PartDev<T>& pd, hipStream_t s)
 {
-    const int nx = pd[0].n, ny = pd[1].n, nz = pd[2].n;
-    const long sxy = static_cast<long>(nx) * ny;
-    const int rows = ny * nz, ylines = nx * nz, zlines = nx * ny;
+    auto kr = k_part_rows<T, KL, SM, KMAX, TILE>;
+    const size_t lds = rows_lds<TILE, T, KL>(pd);
+    const int block = kW * pd.K, ng = (rows + kW - 1) / kW;
     int grid = 0;
-    {
-        auto kr = k_part_rows<T, KL, kPartSeg, KMAX, TILE>;
-        const size_t lds = rows_lds<TILE, T, KL>(pd[0]);
-        const int block = kW * pd[0].K, ng = (rows + kW - 1) / kW;
-        if (hipError_t e = persistent_grid(kr, block, lds, ng, &grid)) return e;
-        hipLaunchKernelGGL(kr, dim3(grid), dim3(block), lds, s, F, rows, pd[0]);
-    }
-    auto kl = k_part_lines<T, KL, kPartSeg, KMAX>;
-    const PartDev<T>& py = pd[1];
-    const PartDev<T>& pz = pd[2];
-    if (hipError_t e = persistent_grid(kl, kW * py.K, scan_lds<T, KL>(py), (ylines + kW - 1) / kW, &grid)) return e;
-    hipLaunchKernelGGL(kl, dim3(grid), dim3(kW * py.K), (scan_lds<T, KL>(py)), s, F, nx, sxy, static_cast<long>(nx),
-                       ylines, py);
-    if (hipError_t e = persistent_grid(kl, kW * pz.K, scan_lds<T, KL>(pz), (zlines + kW - 1) / kW, &grid)) return e;
-    hipLaunchKernelGGL(kl, dim3(grid), dim3(kW * pz.K), (scan_lds<T, KL>(pz)), s, F, zlines, 0L, sxy, zlines, pz);
+    if (hipError_t e = persistent_grid(kr, block, lds, ng, &grid)) return e;
+    hipLaunchKernelGGL(kr, dim3(grid), dim3(block), lds, s, F, rows, pd);
     return hipGetLastError();
+}
+
+template <typename T, int KL, int SM, int KMAX>
+hipError_t part_lines_launch(T* F, int ni, long stride_j, long stride_k, int nlines, const PartDev<T>& pd,
+                             hipStream_t s)
+{
+    auto kl = k_part_lines<T, KL, SM, KMAX>;
+    const size_t lds = scan_lds<T, KL>(pd);
+    int grid = 0;
+    if (hipError_t e = persistent_grid(kl, kW * pd.K, lds, (nlines + kW - 1) / kW, &grid)) return e;
+    hipLaunchKernelGGL(kl, dim3(grid), dim3(kW * pd.K), lds, s, F, ni, stride_j, stride_k, nlines, pd);
+    return hipGetLastError();
+}
+
+// f(integral_constant SM, integral_constant KMAX) for a direction's tables
+template <typename T, class Fn>
+hipError_t part_instance(const PartDev<T>& pd, Fn&& f)
+{
+    if (pd.S == kPartSeg && pd.K <= kPipeMaxK)
+        return f(std::integral_constant<int, kPartSeg>{}, std::integral_constant<int, kPipeMaxK>{});
+    if (pd.S == kPartSeg) return f(std::integral_constant<int, kPartSeg>{}, std::integral_constant<int, kPartMaxK>{});
+    return f(std::integral_constant<int, kPartSegLong>{}, std::integral_constant<int, kPartMaxK>{});
 }
 
 template <typename T, int KL>
 hipError_t part_p(T* F, const PartDev<T> pd[3], hipStream_t s)
 {
     if (!part_planes_fit(pd[0].n, pd[1].n, sizeof(T))) return hipErrorInvalidValue;
-    int kmax = 0;
     for (int d = 0; d < 3; ++d) {
-        if (pd[d].K < 1 || pd[d].K > kPartMaxK || pd[d].S != kPartSeg || (pd[d].K - 1) * pd[d].S >= pd[d].n)
+        if (pd[d].K < 1 || pd[d].K > kPartMaxK || pd[d].S != part_seg(pd[d].n) ||
+            (pd[d].K - 1) * pd[d].S >= pd[d].n)
             return hipErrorInvalidValue;
-        kmax = pd[d].K > kmax ? pd[d].K : kmax;
     }
-    // the kernels' workgroup bound covers the longest direction's partitions
-    if (kmax <= kPipeMaxK) return part_launch<T, KL, kPipeMaxK, 8>(F, pd, s);
-    return part_launch<T, KL, kPartMaxK, 8>(F, pd, s);
+    const int nx = pd[0].n, ny = pd[1].n, nz = pd[2].n;
+    const long sxy = static_cast<long>(nx) * ny;
+    hipError_t e = part_instance(pd[0], [&](auto sm, auto kmax) {
+        return part_rows_launch<T, KL, decltype(sm)::value, decltype(kmax)::value, 8>(F, ny * nz, pd[0], s);
+    });
+    if (e != hipSuccess) return e;
+    e = part_instance(pd[1], [&](auto sm, auto kmax) {
+        return part_lines_launch<T, KL, decltype(sm)::value, decltype(kmax)::value>(F, nx, sxy, static_cast<long>(nx),
+                                                                                    nx * nz, pd[1], s);
+    });
+    if (e != hipSuccess) return e;
+    return part_instance(pd[2], [&](auto sm, auto kmax) {
+        return part_lines_launch<T, KL, decltype(sm)::value, decltype(kmax)::value>(F, nx * ny, 0L, sxy, nx * ny,
+                                                                                    pd[2], s);
+    });
 }
 
 }  // namespace

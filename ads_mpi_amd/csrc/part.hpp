@@ -57,9 +57,23 @@ struct PartDev {
 };
 
 // Entries per partition (a lane keeps its partition in registers) and most
-// partitions per line; longer lines use the LU sweeps.
+// partitions per line (one wave each, so a workgroup of at most 1024
+// threads).  Lines of up to kPartMaxK * kPartSeg entries take partitions of
+// kPartSeg; longer ones, up to kPartMaxK * kPartSegLong, partitions of
+// kPartSegLong (their kernels do not double-buffer, which leaves the
+// registers for the longer partition); longer lines use the LU sweeps.
 constexpr int kPartSeg = 24;
+constexpr int kPartSegLong = 32;
 constexpr int kPartMaxK = 16;
+constexpr int kPartMaxLine = kPartMaxK * kPartSegLong;
+
+// the partition length for a line of n entries (0: too long for the kernels)
+inline int part_seg(int n)
+{
+    if (n <= kPartMaxK * kPartSeg) return kPartSeg;
+    if (n <= kPartMaxLine) return kPartSegLong;
+    return 0;
+}
 
 // The kernels address the 64 lines of a group from its first line with
 // 32-bit byte offsets.  The y lines of a group span at most 64/nx + 1

@@ -211,8 +211,9 @@ int ads_last_drained(ads_ctx* ctx, double* drained);
 /* The algorithm of the FP64 mass solves (ADS_SOLVER_*; no reference
  * counterpart: the reference's MUMPS solves one column at a time).  AUTO
  * (the default) takes the partitioned solve when every direction's lines
- * have at most 384 entries (16 partitions of 24) and an x-y plane stays
- * below 4 GiB; PARTITIONED is refused beyond that. */
+ * have at most 512 entries (16 partitions of 24, or of 32 for lines longer
+ * than 384) and an x-y plane stays below 4 GiB; PARTITIONED is refused
+ * beyond that. */
 int ads_set_solver(ads_ctx* ctx, int solver);
 
 /* Form3DRHS after FormUn (projection_engine.F90:154-284) on the context's
@@ -289,7 +290,8 @@ int ads_host_seg_tables(int n, int p, int* w, int* lo, int* hi, float* tab);
 
 /* Host only: the tables of the partitioned FP64 solve for one direction of
  * SIZE n, degree p (no reference counterpart): the line's n + 1 entries cut
- * into *K partitions of *S (the last may be shorter); rows j = 0 .. K*S + p
+ * into *K partitions of *S (the last may be shorter; S = 24, or 32 for lines
+ * of 385 to 512 entries; longer lines: ADS_ERR_UNSUPPORTED); rows j = 0 .. K*S + p
  * (zero beyond n): lrow[j*p + t] = L(j, j-1-t), urow[j*(p+1)] = 1/U(j, j) and
  * urow[j*(p+1) + t] = U(j, j+t) of the no-interchange LU of the mass matrix,
  * gsp[j*p + i] / hsp[j*p + i] the forward / backward recurrence's response at

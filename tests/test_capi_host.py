@@ -186,6 +186,18 @@ def part_tables(L, n, p):
     return K, S, lrow.reshape(rows, p), urow.reshape(rows, p + 1), gsp.reshape(rows, p), hsp.reshape(rows, p)
 
 
+def test_part_tables_line_limit(L):
+    """Lines of 16 partitions: 384 entries take partitions of 24, 385 and 512
+    partitions of 32, 513 are refused (the LU sweeps serve them)."""
+    for n, S_exp, K_exp in ((383, 24, 16), (384, 32, 13), (511, 32, 16)):
+        K, S = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(L.ads_host_part_tables(n, 3, ctypes.byref(K), ctypes.byref(S), None, None, None, None))
+        assert S.value == S_exp and K.value == K_exp
+    K, S = ctypes.c_int(0), ctypes.c_int(0)
+    assert L.ads_host_part_tables(512, 3, ctypes.byref(K), ctypes.byref(S), None, None, None,
+                                  None) == _lib.ADS_ERR_UNSUPPORTED
+
+
 def part_emulate(K, S, lrow, urow, gsp, hsp, p, b):
     """float64 restatement of the partitioned kernels (k_part_lines): every
     partition from zero state; the two edge-state scans, which take the
@@ -242,14 +254,14 @@ def test_part_tables_and_partitioned_solve(orc, L, n, p):
     (independently assembled) mass matrix M and a dense solve:
       * lrow / urow are LU factors of M without interchanges: L U = M to
         rounding;
-      * partitions of 24 entries, the last one padded past the line with
-        zero rows;
+      * partitions of 24 entries (32 for lines of 385 to 512 entries), the
+        last one padded past the line with zero rows;
       * the partitioned solve (restated in float64 with the tables) equals
         the dense solve of M to rounding (no truncation) for smooth and
         random right-hand sides."""
     N = n + 1
     K, S, lrow, urow, gsp, hsp = part_tables(L, n, p)
-    assert S == 24 and (K - 1) * S < N <= K * S
+    assert S == (24 if N <= 384 else 32) and K <= 16 and (K - 1) * S < N <= K * S
     U = orc.knots(n, p)
     M = orc.band_to_dense(orc.mkbbt_band(U, p, n, n - p + 1, (1, 0, 0, 0)), p, p)
     Lm, Um = np.eye(N), np.zeros((N, N))

@@ -123,8 +123,8 @@ def test_solve_vs_oracle(orc, n, p):
     ((3,3,3) p=3 and (20..22) p=4) and lines whose factor tables do not fit
     in a workgroup's LDS (901 DOFs at p = 3, 1101 at p = 4, 1501 at p = 2,
     2001 at p = 1: tables read from global memory), and the partitioned
-    solve (no-interchange LU, lines cut into partitions of 24 entries)
-    where the lines have at most 16 x 24 = 384 entries."""
+    solve (no-interchange LU, lines cut into partitions of 24 entries, or
+    32 beyond 384) where the lines have at most 16 x 32 = 512 entries."""
     say("case solve", n, p)
     S = orc.Setup(n, p)
     C = Ctx(n, p)
@@ -132,7 +132,7 @@ def test_solve_vs_oracle(orc, n, p):
         F = np.random.default_rng(8).standard_normal(S.D)
         ref, ref_t = S.solve(F), S.solve_T(F)
         solvers = [_lib.ADS_SOLVER_LU]
-        if max(n) < 384:
+        if max(n) < 512:
             solvers.append(_lib.ADS_SOLVER_PARTITIONED)
         else:
             assert C.L.ads_set_solver(C.h, _lib.ADS_SOLVER_PARTITIONED) == _lib.ADS_ERR_UNSUPPORTED
@@ -240,12 +240,14 @@ def test_heat_steps_128cubed_p2():
         adss.Cleanup(ads, data)
 
 
-@pytest.mark.parametrize("n,p", [((129, 129, 129), 2), ((258, 66, 40), 3), ((60, 300, 30), 3), ((33, 20, 383), 4)])
+@pytest.mark.parametrize("n,p", [((129, 129, 129), 2), ((258, 66, 40), 3), ((60, 300, 30), 3), ((33, 20, 383), 4),
+                                 ((400, 40, 30), 3), ((30, 460, 20), 2), ((20, 14, 509), 4)])
 def test_step_both_solvers_vs_oracle(orc, n, p):
     """One FP64 heat step with a lognormal diffusivity by each FP64 mass solve
     (LU sweeps, partitioned) against the oracle, and the two against each
-    other; anisotropic line lengths up to 384 (partitions with and without
-    the double-buffered loads: at most 12 and 16 partitions)."""
+    other; anisotropic line lengths up to 510 (partitions of 24 with and
+    without the double-buffered loads, at most 12 and 16 of them, and of 32
+    for lines of 385 to 512 entries: x rows, y and z lines)."""
     say("case step both solvers", n, p)
     S = orc.Setup(n, p)
     C = Ctx(n, p)
