@@ -235,7 +235,6 @@ def main():
         args.no_cpu_baseline = True
         args.no_grad = True
         args.no_shard = True
-        args.clock_warmup_s = 0.0
         args.steps = min(args.steps, 5)
         args.warmup = min(args.warmup, 1)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
