@@ -395,7 +395,7 @@ __device__ __forceinline__ void wave_lds_sync()
 // coefficient ring.  kHeatDual: GM from c (values), GM and GS from cn
 // (gradients) for the stiffness part.
 template <typename T, int P, int MODE>
-__global__ __launch_bounds__(kWave* kRowsWg) void k_heat_pf(const double* __restrict__ c, T* __restrict__ F,
+__global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k_heat_pf(const double* __restrict__ c, T* __restrict__ F,
                                                             const T* __restrict__ emx, const T* __restrict__ emy,
                                                             const T* __restrict__ emz,
                                                             const double* __restrict__ kappa, T ndt, AsmDims dm,
@@ -407,6 +407,7 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat_pf(const double* __rest
     constexpr int NTH = kWave * kRowsWg;
     constexpr bool GRAD = MODE == kHeatGrad, DUAL = MODE == kHeatDual, TWO = GRAD || DUAL;
     constexpr int NV = DUAL ? 3 : 2;  // exchanged values per position: GM(c) [, GM(cn)], GS
+    constexpr int NHC = kRowsWg * KL * P;  // halo (row, position, azo) triples per element layer
     using A = double;
     __shared__ T cs[P][R][W];                            // coefficient planes, ring by z
     __shared__ A fs[P][R][W];                            // right-hand-side planes, ring by z
@@ -562,15 +563,35 @@ __global__ __launch_bounds__(kWave* kRowsWg) void k_heat_pf(const double* __rest
         int slot[P];
 #pragma unroll
         for (int az = 0; az < P; ++az) slot[az] = (ez + az) % P;
-        // the halo positions' values (rows of every wave; this wave's planes)
-        for (int azo = wv; azo < P; azo += kRowsWg) {
-            T v[P][NV];
-            column_values(hr, kWave + hh, Ez, &eyl[hr][0], &eyl[hr][NT], azo, slot, v);
-            if (hact) {
+        // the halo positions' values.  When the (row, position, azo) triples
+        // fit one wave (p <= 3), one wave computes them all, one triple per
+        // lane, and the wave that does it rotates with the layer and the
+        // workgroup (so no SIMD takes them all); otherwise every wave takes
+        // its planes azo = wv, wv + 4, ... for all rows and positions
+        if constexpr (NHC <= kWave) {
+            if (wv == (ez + static_cast<int>(blockIdx.x)) % kRowsWg) {
+                const bool ha = lane < NHC;
+                const int q = ha ? lane : 0;
+                const int hz = q / (kRowsWg * KL), hrw = (q / KL) % kRowsWg, hp = q % KL;
+                T v[P][NV];
+                column_values(hrw, kWave + hp, Ez, &eyl[hrw][0], &eyl[hrw][NT], hz, slot, v);
+                if (ha) {
 #pragma unroll
-                for (int ayo = 0; ayo < P; ++ayo)
+                    for (int ayo = 0; ayo < P; ++ayo)
 #pragma unroll
-                    for (int k = 0; k < NV; ++k) hal[hr][azo][hh][ayo][k] = v[ayo][k];
+                        for (int k = 0; k < NV; ++k) hal[hrw][hz][hp][ayo][k] = v[ayo][k];
+                }
+            }
+        } else {
+            for (int azo = wv; azo < P; azo += kRowsWg) {
+                T v[P][NV];
+                column_values(hr, kWave + hh, Ez, &eyl[hr][0], &eyl[hr][NT], azo, slot, v);
+                if (hact) {
+#pragma unroll
+                    for (int ayo = 0; ayo < P; ++ayo)
+#pragma unroll
+                        for (int k = 0; k < NV; ++k) hal[hr][azo][hh][ayo][k] = v[ayo][k];
+                }
             }
         }
         __syncthreads();
