@@ -408,6 +408,8 @@ __global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k
     constexpr bool GRAD = MODE == kHeatGrad, DUAL = MODE == kHeatDual, TWO = GRAD || DUAL;
     constexpr int NV = DUAL ? 3 : 2;  // exchanged values per position: GM(c) [, GM(cn)], GS
     constexpr int NHC = kRowsWg * KL * P;  // halo (row, position, azo) triples per element layer
+    constexpr int NHW = (NHC + kWave - 1) / kWave;  // waves that compute them
+    static_assert(NHW <= kRowsWg, "the halo triples fit the workgroup's waves");
     using A = double;
     __shared__ T cs[P][R][W];                            // coefficient planes, ring by z
     __shared__ A fs[P][R][W];                            // right-hand-side planes, ring by z
@@ -444,9 +446,6 @@ __global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k
         const int r = i / (2 * NT);
         (&eyl[0][0])[i] = emy[static_cast<size_t>(min(ey0 + r, dm.ney - 1)) * 2 * NT + i % (2 * NT)];
     }
-    // halo role: lane = (row hr, position 64 + hh)
-    const bool hact = lane < KL * kRowsWg;
-    const int hr = hact ? lane / KL : 0, hh = hact ? lane % KL : 0;
 
     for (int i = tid; i < P * R * W; i += NTH) (&fs[0][0][0])[i] = A(0);
 
@@ -563,15 +562,16 @@ __global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k
         int slot[P];
 #pragma unroll
         for (int az = 0; az < P; ++az) slot[az] = (ez + az) % P;
-        // the halo positions' values.  When the (row, position, azo) triples
-        // fit one wave (p <= 3), one wave computes them all, one triple per
-        // lane, and the wave that does it rotates with the layer and the
-        // workgroup (so no SIMD takes them all); otherwise every wave takes
-        // its planes azo = wv, wv + 4, ... for all rows and positions
-        if constexpr (NHC <= kWave) {
-            if (wv == (ez + static_cast<int>(blockIdx.x)) % kRowsWg) {
-                const bool ha = lane < NHC;
-                const int q = ha ? lane : 0;
+        // the halo positions' values: the (row, position, azo) triples of the
+        // layer, one per lane, on ceil(triples / 64) waves (one at p <= 3, two
+        // at p = 4); which waves take them rotates with the layer and the
+        // workgroup, so that no SIMD takes them all
+        {
+            const int hw = (wv + kRowsWg - (ez + static_cast<int>(blockIdx.x)) % kRowsWg) % kRowsWg;
+            if (hw < NHW) {
+                const int q0 = hw * kWave + lane;
+                const bool ha = q0 < NHC;
+                const int q = ha ? q0 : 0;
                 const int hz = q / (kRowsWg * KL), hrw = (q / KL) % kRowsWg, hp = q % KL;
                 T v[P][NV];
                 column_values(hrw, kWave + hp, Ez, &eyl[hrw][0], &eyl[hrw][NT], hz, slot, v);
@@ -580,17 +580,6 @@ __global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k
                     for (int ayo = 0; ayo < P; ++ayo)
 #pragma unroll
                         for (int k = 0; k < NV; ++k) hal[hrw][hz][hp][ayo][k] = v[ayo][k];
-                }
-            }
-        } else {
-            for (int azo = wv; azo < P; azo += kRowsWg) {
-                T v[P][NV];
-                column_values(hr, kWave + hh, Ez, &eyl[hr][0], &eyl[hr][NT], azo, slot, v);
-                if (hact) {
-#pragma unroll
-                    for (int ayo = 0; ayo < P; ++ayo)
-#pragma unroll
-                        for (int k = 0; k < NV; ++k) hal[hr][azo][hh][ayo][k] = v[ayo][k];
                 }
             }
         }
