@@ -116,7 +116,8 @@ def test_assembly_vs_oracle(orc, n, p):
 
 @pytest.mark.parametrize("n,p", [((9, 11, 7), 2), ((6, 7, 8), 3), ((3, 3, 3), 3), ((20, 21, 22), 4),
                                  ((5, 4, 6), 1), ((70, 5, 66), 2), ((33, 40, 3), 3),
-                                 ((900, 5, 4), 3), ((4, 1100, 5), 4), ((6, 5, 1500), 2), ((2000, 3, 3), 1)])
+                                 ((900, 5, 4), 3), ((4, 1100, 5), 4), ((6, 5, 1500), 2), ((2000, 3, 3), 1),
+                                 ((450, 7, 5), 1), ((5, 6, 420), 4), ((9, 511, 8), 2)])
 def test_solve_vs_oracle(orc, n, p):
     """The FP64 mass solve against the oracle's dgbtrf/dgbtrs ('N' and 'T'):
     the LU substitution sweeps, including cases whose factorisation pivots
