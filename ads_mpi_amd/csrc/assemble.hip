@@ -386,11 +386,13 @@ __device__ __forceinline__ void wave_lds_sync()
 //   F_e(axo, q) = sum_b Mx(axo,b) GM[e+b](q)
 //                 - dt kappa_e (sum_b Kx(axo,b) GM[e+b](q) + sum_b Mx(axo,b) GS[e+b](q))
 //   l2_e = sum_{axo,q} c(e+axo, q) sum_b Mx(axo,b) GM[e+b](q)
-// 5 P^3 + 3.5 P^4 multiply-adds per element against 8 P^4 + 2 P^3.  A lane
-// computes GM, GS of its own position; the p positions past the wave's 64
-// (the halo) are computed by the waves together (wave w: azo = w, w+4, ...;
-// lanes = (row, position)) into LDS before a barrier.  Neighbouring
-// positions' values go through a per-wave LDS slot, one (ayo, azo) at a time.
+// 5 P^3 (y, z) + 3 P^4 (x) + 2 P^3 (combine, l2) multiply-adds per element
+// against 8 P^4 + 2 P^3.  A lane computes GM, GS of its own position; the p
+// positions past the wave's 64 (the halo) of every row go into LDS before a
+// barrier, one (row, position, azo) triple per lane on one or two waves.
+// Neighbouring positions' values go through a per-wave LDS slot, one (ayo,
+// azo) at a time.  The plain instances up to p = 3 are held to 4 waves per
+// SIMD (the halo pass would otherwise take them to 3).
 // kHeatGrad: c is mu, and dkappa[e] += -dt cn_e^T (K_e mu_e) from a second
 // coefficient ring.  kHeatDual: GM from c (values), GM and GS from cn
 // (gradients) for the stiffness part.
