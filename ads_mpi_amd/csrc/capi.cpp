@@ -1491,7 +1491,7 @@ int ads_matrix_solve(ads_matrix* m, double* rhs, int eqnum)
     };
     if (eqnum == 0) return ADS_OK;
     const int N = m->dir.dofs, p = m->dir.p;
-    if (p > 4) return mfail(ADS_ERR_UNSUPPORTED, "ads_matrix_solve: the GPU sweeps are built for degrees 1..4");
+    if (p > 9) return mfail(ADS_ERR_UNSUPPORTED, "ads_matrix_solve: the GPU sweeps are built for degrees 1..9");
     hipError_t e = hipSuccess;
     if (!m->stream) {
         std::vector<double> fwd, bwd;

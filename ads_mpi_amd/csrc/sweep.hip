@@ -397,7 +397,10 @@ __device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int ro
     constexpr int RPI = kWave / CH;      // rows per cooperative instruction
     constexpr int NIT = kWave / RPI;     // cooperative instructions per tile
     using PS = PASS<T, KL, PIV>;
-    static_assert(CH % kGroup == 0, "tile of whole groups");
+    // factor entries loaded ahead: kGroup steps, fewer for the wide bands of
+    // degrees above 4 (SolveOneDirection only), whose tables would spill
+    constexpr int GR = KL <= 4 ? kGroup : 2;
+    static_assert(CH % GR == 0, "tile of whole groups");
     const long rowlen = n;
     const int myrow = row0 + lane;
     const bool active = myrow < nrows;
@@ -430,31 +433,31 @@ __device__ __forceinline__ void run_rows(T* __restrict__ data, int nrows, int ro
         wave_lds_sync();
         if (jlo >= 0 && jlo + CH <= n) {
 #pragma unroll
-            for (int g = 0; g < CH; g += kGroup) {
-                GroupTabs<PS, T, kGroup> gt;
+            for (int g = 0; g < CH; g += GR) {
+                GroupTabs<PS, T, GR> gt;
 #pragma unroll
-                for (int i = 0; i < kGroup; ++i) {
+                for (int i = 0; i < GR; ++i) {
                     const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
                     PS::load(jlo + c, fwd, bwd, piv, gt.tb[i], gt.l[i]);
                 }
 #pragma unroll
-                for (int i = 0; i < kGroup; ++i) {
+                for (int i = 0; i < GR; ++i) {
                     const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
                     tile[lane * LDT + c] = ps.step(tile[lane * LDT + c], gt.tb[i], gt.l[i]);
                 }
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < CH; g += kGroup) {
-                GroupTabs<PS, T, kGroup> gt;
+            for (int g = 0; g < CH; g += GR) {
+                GroupTabs<PS, T, GR> gt;
 #pragma unroll
-                for (int i = 0; i < kGroup; ++i) {
+                for (int i = 0; i < GR; ++i) {
                     const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
                     const int j = jlo + c;
                     if (j >= 0 && j < n) PS::load(j, fwd, bwd, piv, gt.tb[i], gt.l[i]);
                 }
 #pragma unroll
-                for (int i = 0; i < kGroup; ++i) {
+                for (int i = 0; i < GR; ++i) {
                     const int c = PS::kAscending ? g + i : CH - 1 - (g + i);
                     const int j = jlo + c;
                     if (j >= 0 && j < n) tile[lane * LDT + c] = ps.step(tile[lane * LDT + c], gt.tb[i], gt.l[i]);
@@ -1242,6 +1245,12 @@ hipError_t launch_rows_solve(int p, double* F, int nrows, const SweepTables<doub
     case 2: return rows_solve_p<double, 2>(F, nrows, st, trans, s);
     case 3: return rows_solve_p<double, 3>(F, nrows, st, trans, s);
     case 4: return rows_solve_p<double, 4>(F, nrows, st, trans, s);
+    // degrees beyond the step's (SolveOneDirection on a caller's matrix)
+    case 5: return rows_solve_p<double, 5>(F, nrows, st, trans, s);
+    case 6: return rows_solve_p<double, 6>(F, nrows, st, trans, s);
+    case 7: return rows_solve_p<double, 7>(F, nrows, st, trans, s);
+    case 8: return rows_solve_p<double, 8>(F, nrows, st, trans, s);
+    case 9: return rows_solve_p<double, 9>(F, nrows, st, trans, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -346,7 +346,8 @@ int ads_matrix_dense(const ads_matrix* m, double* dense);
 /* SolveOneDirection(RHS, eqnum, n, p, sprsmtrx) -- src/ADS.F90:294-379:
  * rhs (host, column-major (n+1) x eqnum, each column one right-hand side) is
  * overwritten by the solutions, computed on the GPU (one lane per column, the
- * LU substitution sweeps; p = 1..4).  Copies in and out and synchronises. */
+ * LU substitution sweeps; p = 1..9, as ads_matrix_create).  Copies in and out
+ * and synchronises. */
 int ads_matrix_solve(ads_matrix* m, double* rhs, int eqnum);
 
 /* ---- host-only helpers (no GPU needed) ----------------------------------- */

@@ -121,10 +121,12 @@ def graded_knots(n, p, power=1.5):
 
 @pytest.mark.parametrize("n,p,eqnum,graded", [(17, 2, 324, False), (40, 3, 1000, True), (3, 3, 16, False),
                                               (20, 4, 77, False), (9, 1, 5, True), (1500, 3, 300, False),
-                                              (130, 2, 16900, False)])
+                                              (130, 2, 16900, False), (30, 5, 40, False), (300, 6, 64, True),
+                                              (45, 7, 120, True), (40, 8, 7, False), (24, 9, 33, False)])
 def test_compute_matrix_and_solve_one_direction(orc, n, p, eqnum, graded):
     """SolveOneDirection with the matrix of ComputeMatrix, for the mass
-    matrix and non-symmetric mixes, on the GPU: against dense solves."""
+    matrix and non-symmetric mixes, on the GPU: against dense solves; degrees
+    1..9 (the reference's Gauss rules reach Q = 10), above the step's 1..4."""
     say("case SolveOneDirection", n, p, eqnum, graded)
     U = graded_knots(n, p) if graded else orc.knots(n, p)
     rng = np.random.default_rng(n + p)
