@@ -1267,8 +1267,10 @@ template <typename T, int P, int MODE>
 hipError_t heat_sel(const double* c, T* F, const Tables<T>& tb, double dt, const double* kappa, double* l2_part,
                     int part_cap, int* nparts, const double* cn, double* dkappa, hipStream_t s)
 {
-    if (ADS_HEAT_PF) return heat_pf_p<T, P, MODE>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, cn, dkappa, s);
-    return heat_p<T, P, MODE>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, cn, dkappa, s);
+    if constexpr (ADS_HEAT_PF != 0)
+        return heat_pf_p<T, P, MODE>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, cn, dkappa, s);
+    else
+        return heat_p<T, P, MODE>(c, F, tb, dt, kappa, l2_part, part_cap, nparts, cn, dkappa, s);
 }
 
 template <typename T, int P>
@@ -1364,6 +1366,7 @@ hipError_t launch_assemble(int p, int form, int f0, const double* c, T* F, const
     case 2: return assemble_p<T, 3>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
     case 3: return assemble_p<T, 4>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
     case 4: return assemble_p<T, 5>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
+    case 5: return assemble_p<T, 6>(form, f0, c, F, tb, dt, kappa, l2_part, part_cap, nparts, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1384,6 +1387,7 @@ hipError_t launch_eval_grid(int p, const double* c, int sx, int sy, int sz, cons
     case 2: hipLaunchKernelGGL(k_eval_grid<3>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
     case 3: hipLaunchKernelGGL(k_eval_grid<4>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
     case 4: hipLaunchKernelGGL(k_eval_grid<5>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
+    case 5: hipLaunchKernelGGL(k_eval_grid<6>, grid, dim3(256), 0, s, c, sx, sy, sz, fx, bx, rx, fy, by, ry, fz, bz, rz, out); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1407,6 +1411,7 @@ hipError_t launch_heat_adjoint(int p, const double* mu, const double* c_n, T* la
     case 2: return heat_sel<T, 3, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
     case 3: return heat_sel<T, 4, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
     case 4: return heat_sel<T, 5, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
+    case 5: return heat_sel<T, 6, kHeatGrad>(mu, lam_out, tb, dt, kappa, l2_part, part_cap, nparts, c_n, dkappa, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1419,6 +1424,7 @@ hipError_t launch_heat_dual(int p, const double* c_val, const double* c_grad, do
     case 2: return heat_sel<double, 3, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
     case 3: return heat_sel<double, 4, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
     case 4: return heat_sel<double, 5, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
+    case 5: return heat_sel<double, 6, kHeatDual>(c_val, F, tb, dt, kappa, l2_part, part_cap, nparts, c_grad, nullptr, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -711,8 +711,8 @@ int ads_init(const int n[3], const int p[3], ads_ctx** out)
     if (!n || !p || !out) return fail(nullptr, ADS_ERR_INVALID, "null argument");
     *out = nullptr;
     for (int d = 0; d < 3; ++d)
-        if (p[d] < 1 || p[d] > 4)
-            return fail(nullptr, ADS_ERR_UNSUPPORTED, "degree must be 1..4 in each direction for the GPU kernels");
+        if (p[d] < 1 || p[d] > 5)
+            return fail(nullptr, ADS_ERR_UNSUPPORTED, "degree must be 1..5 in each direction for the GPU kernels");
     ads_ctx* ctx = new (std::nothrow) ads_ctx();
     if (!ctx) return fail(nullptr, ADS_ERR_NOMEM, "host allocation failed");
     ctx->p = std::max(p[0], std::max(p[1], p[2]));
@@ -1207,6 +1207,7 @@ int ads_oil_setup(ads_ctx* ctx, const double* cx, const double* cy, const double
     if (params[3] <= 0.0) return fail(ctx, ADS_ERR_INVALID, "ads_oil_setup: radius must be positive");
     if (ctx->dir[0].p != ctx->dir[1].p || ctx->dir[1].p != ctx->dir[2].p)
         return fail(ctx, ADS_ERR_UNSUPPORTED, "ads_oil_setup: the oil form needs the same degree in x, y and z");
+    if (ctx->p > 4) return fail(ctx, ADS_ERR_UNSUPPORTED, "ads_oil_setup: the oil kernels are built for degrees 1..4");
     auto& o = ctx->oil;
     o.ready = false;
     const long npt = static_cast<long>(cN) * cL;
@@ -1275,7 +1276,7 @@ int ads_oil_get_kq(ads_ctx* ctx, double* kq_host)
 
 int ads_host_seg_tables(int n, int p, int* w, int* lo, int* hi, float* tab)
 {
-    if (!w || !lo || !hi || !tab || n < p || p < 1 || p > 4) return ADS_ERR_INVALID;
+    if (!w || !lo || !hi || !tab || n < p || p < 1 || p > 5) return ADS_ERR_INVALID;
     ads::Direction d;
     if (!ads::build_direction(n, p, d).empty()) return ADS_ERR_INVALID;
     std::vector<float> t;

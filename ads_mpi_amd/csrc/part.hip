@@ -547,6 +547,7 @@ hipError_t launch_part_solve(int p, T* F, const PartDev<T> pd[3], hipStream_t s)
     case 2: return part_p<T, 2>(F, pd, s);
     case 3: return part_p<T, 3>(F, pd, s);
     case 4: return part_p<T, 4>(F, pd, s);
+    case 5: return part_p<T, 5>(F, pd, s);
     default: return hipErrorInvalidValue;
     }
 }

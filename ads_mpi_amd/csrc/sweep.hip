@@ -811,6 +811,7 @@ hipError_t seg_dispatch(int p, int w, F&& f)
     case 2: return by_w(std::integral_constant<int, 2>{});
     case 3: return by_w(std::integral_constant<int, 3>{});
     case 4: return by_w(std::integral_constant<int, 4>{});
+    case 5: return by_w(std::integral_constant<int, 5>{});
     default: return hipErrorInvalidValue;
     }
 }
@@ -1233,6 +1234,7 @@ hipError_t launch_kron_solve(int p, T* F, const SweepTables<T> sw[3], bool trans
     case 2: return kron_solve_p<T, 2>(F, sw, trans, s);
     case 3: return kron_solve_p<T, 3>(F, sw, trans, s);
     case 4: return kron_solve_p<T, 4>(F, sw, trans, s);
+    case 5: return kron_solve_p<T, 5>(F, sw, trans, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1282,8 +1284,8 @@ template hipError_t launch_kron_solve<float>(int, float*, const SweepTables<floa
 
 long kron_residual_partials_bound(const BandSet& bs)
 {
-    // the narrowest output tile (p = 4) gives the most x chunks
-    const int ow = res_out_width(4);
+    // the narrowest output tile (p = 5) gives the most x chunks
+    const int ow = res_out_width(5);
     return static_cast<long>((bs.n[0] + ow - 1) / ow) * ((bs.n[1] + kResRows - 1) / kResRows) * bs.n[2];
 }
 
@@ -1307,6 +1309,7 @@ hipError_t launch_kron_residual(int p, const double* c, const TX* x, const doubl
     case 2: return residual_p<3, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
     case 3: return residual_p<4, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
     case 4: return residual_p<5, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
+    case 5: return residual_p<6, TX, TR>(c, x, b, r, bs, dtk, l2_part, part_cap, nparts, bo, s);
     default: return hipErrorInvalidValue;
     }
 }

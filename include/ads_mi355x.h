@@ -76,7 +76,7 @@ typedef struct ads_ctx ads_ctx;
 
 /* ADSS::initialize(n, p, ads, ads_data, mierr) -- src/ADS.F90:11-107.
  * n = number of basis functions minus one per direction (the heat CLI SIZE),
- * p = B-spline degree per direction (1..4 each; directions may differ, as
+ * p = B-spline degree per direction (1..5 each; directions may differ, as
  * in the reference's p(3): the kernels run at the largest degree on tables
  * widened with exact zeros).
  * Builds the knots (PrepareKnot, knot_vector.F90:43-61), the basis tables

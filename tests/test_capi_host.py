@@ -50,8 +50,8 @@ def test_generators_bitwise_equal_oracle(orc):
 def test_init_argument_checks_fail_loudly(L):
     I3 = ctypes.c_int * 3
     h = ctypes.c_void_p()
-    # degrees 1..4 per direction (unequal ones too); beyond that the kernels are not built
-    assert L.ads_init(I3(9, 9, 9), I3(2, 5, 2), ctypes.byref(h)) == _lib.ADS_ERR_UNSUPPORTED
+    # degrees 1..5 per direction (unequal ones too); beyond that the kernels are not built
+    assert L.ads_init(I3(9, 9, 9), I3(2, 6, 2), ctypes.byref(h)) == _lib.ADS_ERR_UNSUPPORTED
     assert b"degree" in L.ads_last_error(None)
     assert L.ads_init(I3(9, 9, 9), I3(7, 7, 7), ctypes.byref(h)) == _lib.ADS_ERR_UNSUPPORTED
     assert L.ads_init(I3(1, 9, 9), I3(2, 2, 2), ctypes.byref(h)) == _lib.ADS_ERR_INVALID

@@ -27,6 +27,14 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
 
 
+def solved_tol(p):
+    """Bound for results that went through the mass solve.  At p = 5 the
+    solve amplifies rounding differences of the right-hand side (~2e-16) by
+    ~2e4: the oracle's own faithful and sum-factorised steps differ by 5e-12
+    on these meshes, so the bound is ten times that; 1e-12 up to p = 4."""
+    return TOL if int(np.max(p)) <= 4 else 5e-11
+
+
 @pytest.fixture(scope="module", autouse=True)
 def _gpu():
     if not torch.cuda.is_available():
@@ -96,7 +104,7 @@ def test_l2projection_known_answer_gpu():
 
 
 @pytest.mark.parametrize("n,p", [((9, 11, 7), 2), ((6, 7, 8), 3), ((5, 4, 6), 1), ((9, 8, 10), 4),
-                                 ((70, 5, 6), 2), ((131, 9, 4), 3)])
+                                 ((70, 5, 6), 2), ((131, 9, 4), 3), ((10, 9, 11), 5)])
 def test_assembly_vs_oracle(orc, n, p):
     say("case assembly", n, p)
     S = orc.Setup(n, p)
@@ -117,7 +125,8 @@ def test_assembly_vs_oracle(orc, n, p):
 @pytest.mark.parametrize("n,p", [((9, 11, 7), 2), ((6, 7, 8), 3), ((3, 3, 3), 3), ((20, 21, 22), 4),
                                  ((5, 4, 6), 1), ((70, 5, 66), 2), ((33, 40, 3), 3),
                                  ((900, 5, 4), 3), ((4, 1100, 5), 4), ((6, 5, 1500), 2), ((2000, 3, 3), 1),
-                                 ((450, 7, 5), 1), ((5, 6, 420), 4), ((9, 511, 8), 2)])
+                                 ((450, 7, 5), 1), ((5, 6, 420), 4), ((9, 511, 8), 2), ((12, 14, 13), 5),
+                                 ((400, 6, 7), 5)])
 def test_solve_vs_oracle(orc, n, p):
     """The FP64 mass solve against the oracle's dgbtrf/dgbtrs ('N' and 'T'):
     the LU substitution sweeps, including cases whose factorisation pivots
@@ -269,7 +278,8 @@ def test_step_both_solvers_vs_oracle(orc, n, p):
         C.close()
 
 
-@pytest.mark.parametrize("n,p", [((14, 17, 12), (2, 3, 1)), ((9, 20, 16), (4, 2, 3)), ((30, 8, 10), (1, 1, 4))])
+@pytest.mark.parametrize("n,p", [((14, 17, 12), (2, 3, 1)), ((9, 20, 16), (4, 2, 3)), ((30, 8, 10), (1, 1, 4)),
+                                 ((14, 12, 13), (5, 3, 2)), ((11, 12, 10), (5, 5, 5))])
 def test_unequal_degrees_vs_oracle(orc, n, p):
     """A degree per direction (ADSS::initialize(n(3), p(3), ...),
     src/ADS.F90:11): the kernels run at the largest degree on tables widened
@@ -285,7 +295,7 @@ def test_unequal_degrees_vs_oracle(orc, n, p):
         assert list(ads.p) == list(p)
         l2 = adss.Step(0, adss.HeatRHS(Dt=1e-7, f0=adss.ADS_F0_BUMP3D), ads, data)
         c0, l2o = S.project(O.F0_BUMP3D, O.MODE_FAITHFUL)
-        assert rel(data.F, c0) <= TOL and abs(l2 - l2o) <= TOL * l2o
+        assert rel(data.F, c0) <= solved_tol(p) and abs(l2 - l2o) <= TOL * l2o
         C = Ctx.__new__(Ctx)
         C.ads, C.data, C.h, C.L, C.D = ads, data, data.handle, _lib.load(), data.D
         c = np.random.default_rng(51).uniform(-1, 1, S.D)
@@ -294,17 +304,20 @@ def test_unequal_degrees_vs_oracle(orc, n, p):
         exp, _ = S.step(c, dt, kappa=kappa, mode=O.MODE_FAITHFUL)
         for solver in (_lib.ADS_SOLVER_LU, _lib.ADS_SOLVER_PARTITIONED):
             _lib.check(C.L.ads_set_solver(C.h, solver), C.h)
-            assert rel(C.step_d(c, dt, kappa=kappa, prec=0), exp) <= TOL
+            assert rel(C.step_d(c, dt, kappa=kappa, prec=0), exp) <= solved_tol(p)
         exp1, _ = S.step(c, dt)
         ref = C.step_d(c, dt, prec=0)
-        assert rel(ref, exp1) <= TOL
-        assert rel(C.step_d(c, dt, prec=_lib.ADS_PREC_FP64_KRON), exp1) <= TOL
+        assert rel(ref, exp1) <= solved_tol(p)
+        assert rel(C.step_d(c, dt, prec=_lib.ADS_PREC_FP64_KRON), exp1) <= solved_tol(p)
+        # two passes reach 1e-10 up to p = 4; at p = 5 the FP32 solves' error
+        # grows with M's condition (7e-10 after two passes) and a third is taken
+        refine = 2 if max(p) <= 4 else 3
         for prec in (_lib.ADS_PREC_FP32_IR, _lib.ADS_PREC_FP32_IR_KRON, _lib.ADS_PREC_FP32_IR_SEG):
-            assert rel(C.step_d(c, dt, prec=prec, refine=2), ref) <= 1e-10
+            assert rel(C.step_d(c, dt, prec=prec, refine=refine), ref) <= 1e-10
         mixes = [[1.0, 2e-3, 0.0, 0.0], [1.0, 0.0, 0.02, 0.0], [0.5, 1e-3, 0.0, 0.02]]
         data.F = c
         adss.MultiStep(1, np.array(mixes).T, adss.HeatRHS(Dt=1e-4, kappa=kappa), ads, data)
-        assert rel(data.F, S.multistep(c, mixes, 1e-4, kappa=kappa)[0]) <= TOL
+        assert rel(data.F, S.multistep(c, mixes, 1e-4, kappa=kappa)[0]) <= solved_tol(p)
         lam = np.random.default_rng(52).standard_normal(S.D)
         dko = np.zeros(S.E)
         lo = S.step_adjoint(c, lam, dt, kappa=kappa, dkappa=dko)
@@ -313,7 +326,7 @@ def test_unequal_degrees_vs_oracle(orc, n, p):
         torch.cuda.synchronize()
         _lib.check(C.L.ads_step_adjoint_d(C.h, vp(cd), vp(ld), vp(kd), dt, 0, 0, vp(out), vp(dk)), C.h)
         _lib.check(C.L.ads_synchronize(C.h), C.h)
-        assert rel(out.cpu().numpy(), lo) <= TOL and rel(dk.cpu().numpy(), dko) <= TOL
+        assert rel(out.cpu().numpy(), lo) <= solved_tol(p) and rel(dk.cpu().numpy(), dko) <= solved_tol(p)
         # the output grid: the spline at a 9 x 7 x 5 grid, against the oracle's EvalSpline
         data.F = c
         params = adss.PlotParams(resx=9, resy=7, resz=5)
@@ -392,7 +405,7 @@ def test_fp32_refinement_form():
 
 
 @pytest.mark.parametrize("n,p", [((9, 11, 7), 2), ((6, 7, 8), 3), ((5, 4, 6), 1), ((12, 8, 10), 4),
-                                 ((70, 5, 66), 2), ((40, 33, 21), 3)])
+                                 ((70, 5, 66), 2), ((40, 33, 21), 3), ((14, 12, 13), 5)])
 def test_fp64_kron_form_vs_oracle(n, p):
     """ADS_PREC_FP64_KRON: the right-hand side as the Kronecker operator of the
     1-D matrices.  Against the oracle's faithful step (per-point quadrature,
@@ -415,7 +428,7 @@ def test_fp64_kron_form_vs_oracle(n, p):
             ref, l2o = S.step(c, dt, mode=O.MODE_FAITHFUL, kappa=kappa)
             err = rel(data.F, ref)
             say("kappa", kappa is not None, "rel err", err, "l2 err", abs(l2 - l2o) / l2o)
-            assert err <= TOL
+            assert err <= solved_tol(p)
             assert abs(l2 - l2o) <= TOL * l2o
     finally:
         adss.Cleanup(ads, data)
