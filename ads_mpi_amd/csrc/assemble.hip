@@ -883,6 +883,28 @@ __device__ __forceinline__ double oil_sources(const double* __restrict__ q, int 
     return f;
 }
 
+// Whether a source of q can lie within its radius of a point of the box
+// [lo, hi]: the box distance is a lower bound of every point's distance, and
+// the margin keeps the test conservative under rounding, so when it says no,
+// oil_sources returns exactly 0 for every point of the box.
+__device__ __forceinline__ bool oil_sources_near(const double* __restrict__ q, int nq, double radius,
+                                                 const double (&lo)[3], const double (&hi)[3])
+{
+    const double cut = radius * radius * (1.0 + 1e-9);
+    bool near = false;
+    for (int i = 0; i < nq; ++i) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double v = q[3 * i + a];
+            const double gap = fmax(0.0, fmax(lo[a] - v, v - hi[a]));
+            d2 += gap * gap;
+        }
+        near = near || d2 <= cut;
+    }
+    return near;
+}
+
 // initial_state (input_data.F90:317-329): 0.1 lerp(falloff(0, .1, dist), 0, 1) bump3d(.2, .6, X),
 // the distance over the candidate segments of radius 0.1
 __device__ double oil_initial(const OilDev& o, const int* __restrict__ ends, int nlist, double x, double y, double z)
@@ -1035,6 +1057,25 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
         }
         // the element's coefficients: first DOF of element e is e (open knots)
         const double* __restrict__ ce = c ? c + exc + static_cast<long>(sx) * ey + plane * ez : nullptr;
+        // pumps and drains near the element's points (most elements have
+        // none, and their point loops over the sources are skipped)
+        bool pump_near = false, drain_near = false;
+        if constexpr (!TZERO) {
+            const double* pt[3] = {ptsx + static_cast<size_t>(exc) * Q, ptsy + static_cast<size_t>(ey) * Q,
+                                   ptsz + static_cast<size_t>(ez) * Q};
+            double lo[3], hi[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = hi[a] = pt[a][0];
+#pragma unroll
+                for (int q = 1; q < Q; ++q) {
+                    lo[a] = fmin(lo[a], pt[a][q]);
+                    hi[a] = fmax(hi[a], pt[a][q]);
+                }
+            }
+            pump_near = oil_sources_near(o.pumps, o.npumps, o.radius, lo, hi);
+            drain_near = oil_sources_near(o.drains, o.ndrains, o.radius, lo, hi);
+        }
 
 #pragma unroll 1
         for (int qx = 0; qx < Q; ++qx) {
@@ -1124,8 +1165,10 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
                             duz = fma(nz1[az], u00[az], duz);
                         }
                         const double kqv = kq[qx + Q * (qy + Q * (qz + static_cast<long>(Q) * e))];
-                        const double pump = oil_sources(o.pumps, o.npumps, o.pump_strength, o.radius, xq, yq, zq);
-                        const double dw = oil_sources(o.drains, o.ndrains, o.drain_strength, o.radius, xq, yq, zq);
+                        const double pump =
+                            pump_near ? oil_sources(o.pumps, o.npumps, o.pump_strength, o.radius, xq, yq, zq) : 0.0;
+                        const double dw =
+                            drain_near ? oil_sources(o.drains, o.ndrains, o.drain_strength, o.radius, xq, yq, zq) : 0.0;
                         const double vd = fmax(0.0, dw * u);
                         w0 = w * (u + dt * (pump - vd));
                         const double g = -w * dt * kqv * exp(o.mi * u);
