@@ -973,7 +973,9 @@ __global__ __launch_bounds__(256) void k_oil_kq(const double* __restrict__ ptsx,
     __syncthreads();
     const double dist = sqrt(oil_dist_curves(o, ends, nlist, pt[0], pt[1], pt[2]));
     const double t = oil_falloff(0.0, 0.06, dist);
-    if (i0 < n) kq[i0] = (1.0 - t) * o.kq_min + t * o.kq_max;
+    // stored point-major (kq[(kx + Q (ky + Q kz)) E + e]), so the step kernel's
+    // lanes (consecutive elements) read each point's values coalesced
+    if (i0 < n) kq[(kx + Q * (ky + Q * kz)) * (n / (Q * Q * Q)) + e] = (1.0 - t) * o.kq_min + t * o.kq_max;
 }
 
 template <int P, bool TZERO>
@@ -1012,6 +1014,7 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
     const double* __restrict__ Nx = tabx + static_cast<size_t>(exc) * Q * P * 2;
     const double* __restrict__ Ny = taby + static_cast<size_t>(ey) * Q * P * 2;
     const double Jxy = jacx[exc] * jacy[ey];
+    const long ne = static_cast<long>(dm.nex) * dm.ney * dm.nez;  // elements (the kq cache's point stride)
 
     for (int i = lane; i < P * P * W; i += kWave) (&fs[0][0][0])[i] = 0.0;
     auto flush_plane = [&](int z) {
@@ -1164,7 +1167,7 @@ __global__ __launch_bounds__(kWave) void k_oil(const double* __restrict__ c, dou
                             duy = fma(nz0[az], u01[az], duy);
                             duz = fma(nz1[az], u00[az], duz);
                         }
-                        const double kqv = kq[qx + Q * (qy + Q * (qz + static_cast<long>(Q) * e))];
+                        const double kqv = kq[(qx + Q * (qy + Q * qz)) * ne + e];
                         const double pump =
                             pump_near ? oil_sources(o.pumps, o.npumps, o.pump_strength, o.radius, xq, yq, zq) : 0.0;
                         const double dw =

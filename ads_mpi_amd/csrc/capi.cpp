@@ -1268,9 +1268,19 @@ int ads_oil_get_kq(ads_ctx* ctx, double* kq_host)
     if (check_ctx(ctx) || !kq_host) return ADS_ERR_INVALID;
     if (!ctx->oil.ready) return fail(ctx, ADS_ERR_INVALID, "ads_oil_get_kq: call ads_oil_setup first");
     const long q3 = static_cast<long>(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1);
-    ADS_HIP(ctx, hipMemcpyAsync(kq_host, ctx->oil.kq, sizeof(double) * ctx->E * q3, hipMemcpyDeviceToHost,
+    // the device cache is point-major (kq[q E + e]); the caller gets the
+    // reference's Kqvals order (point fastest, then element)
+    std::vector<double> dev;
+    try {
+        dev.resize(static_cast<size_t>(ctx->E) * q3);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, ADS_ERR_NOMEM, "ads_oil_get_kq: host allocation failed");
+    }
+    ADS_HIP(ctx, hipMemcpyAsync(dev.data(), ctx->oil.kq, sizeof(double) * dev.size(), hipMemcpyDeviceToHost,
                                 ctx->stream));
     ADS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (long e = 0; e < ctx->E; ++e)
+        for (long q = 0; q < q3; ++q) kq_host[q + q3 * e] = dev[static_cast<size_t>(q * ctx->E + e)];
     return ADS_OK;
 }
 
