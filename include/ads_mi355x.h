@@ -277,8 +277,11 @@ int ads_oil_project(ads_ctx* ctx, double* l2norm);
  * step's sum J W v Dt vdrain (the reference's module variable `drained`
  * accumulates these over all steps, RHS_eq.F90:86). */
 int ads_oil_step(ads_ctx* ctx, double dt, double* l2norm, double* drained);
-/* the cached kq values, Kqvals(kx, ky, kz, ex, ey, ez) order (kx fastest):
- * E (p+1)^3 doubles */
+/* the cached kq values (PrecomputeKq's Kqvals, problems/oil/input_data.F90:384-395)
+ * in Kqvals(kx, ky, kz, ex, ey, ez) order (kx fastest): E (p+1)^3 doubles.
+ * The device copy is point-major (point index times E plus element), so this
+ * call copies it to the host and permutes it there (a host buffer of the same
+ * size; ADS_ERR_NOMEM when that allocation fails). */
 int ads_oil_get_kq(ads_ctx* ctx, double* kq_host);
 /* Host only: the factor tables of the segmented form for one direction of
  * SIZE n, degree p (no reference counterpart): tab = (n+2) x (2p+1) FP32 rows
