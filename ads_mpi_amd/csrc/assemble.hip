@@ -571,6 +571,9 @@ __global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k
         {
             const int hw = (wv + kRowsWg - (ez + static_cast<int>(blockIdx.x)) % kRowsWg) % kRowsWg;
             if (hw < NHW) {
+                // the other waves of the workgroup wait for this at the next
+                // barrier: issue it ahead of the co-resident workgroups' waves
+                __builtin_amdgcn_s_setprio(2);
                 const int q0 = hw * kWave + lane;
                 const bool ha = q0 < NHC;
                 const int q = ha ? q0 : 0;
@@ -583,6 +586,7 @@ __global__ __launch_bounds__(kWave* kRowsWg, P <= 4 && MODE == 0 ? 4 : 1) void k
 #pragma unroll
                         for (int k = 0; k < NV; ++k) hal[hrw][hz][hp][ayo][k] = v[ayo][k];
                 }
+                __builtin_amdgcn_s_setprio(0);
             }
         }
         __syncthreads();
